@@ -1,0 +1,233 @@
+/*
+ * lbm_amd.h -- C ABI of the MI355X-native D2Q9 lattice-Boltzmann engine.
+ *
+ * This is the drop-in boundary under the LLALBM parallelization-policy API.
+ * A policy class (HIPPolicy<2, Configuration>, see the C++ headers shipped in
+ * the package) forwards the static methods that Lattice::perform_lbm calls to
+ * these entry points.  Populations stay on the GPU between calls.  Host
+ * arrays are owned by the caller and copied.  Every function returns an
+ * lbm_status; lbm_last_error() describes the last failure of the calling
+ * thread.  One host thread drives a context; calls on one context are
+ * stream-ordered and not re-entrant.
+ *
+ * Layout conventions follow the reference's Eigen ColMajor tensors:
+ *   populations (NR, NC, 9): element (i, j, d) at  i + NR*(j + NC*d)
+ *   density     (NR, NC)   : element (i, j)    at  i + NR*j
+ *   velocity    (NR, NC, 2): element (i, j, k) at  i + NR*(j + NC*k)
+ *   node class  (NR, NC)   : uint8 per node, values of lbm_node_class
+ * with i = row (0 = top) and j = column, as in the reference
+ * (llalbm/src/core/lattices/Lattice.hpp:58-70, NodeIdentification.hpp:49-51).
+ */
+#ifndef LBM_AMD_H
+#define LBM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lbm_ctx lbm_ctx;
+
+typedef enum lbm_status {
+    LBM_OK = 0,
+    LBM_ERR_INVALID = 1,     /* bad argument or geometry */
+    LBM_ERR_HIP = 2,         /* HIP runtime failure */
+    LBM_ERR_NAN = 3,         /* a NaN density was produced (reference: LLALBM_DEBUG check, Lattice.hpp:412-422) */
+    LBM_ERR_STATE = 4,       /* call out of order (e.g. step before geometry) */
+    LBM_ERR_UNSUPPORTED = 5, /* valid in the reference, not handled here (documented cases) */
+    LBM_ERR_NOMEM = 6
+} lbm_status;
+
+/* Node classes = the reference's InputNodeType (llalbm/src/utils/aliases.hpp:41-48). */
+typedef enum lbm_node_class {
+    LBM_FLUID = 0,
+    LBM_SOLID = 1,
+    LBM_BOUNDARY = 2,
+    LBM_INLET = 3,
+    LBM_OUTLET = 4,
+    LBM_OBSTACLE = 5
+} lbm_node_class;
+
+typedef enum lbm_collision {
+    LBM_BGK = 0, /* BGKCollision.hpp:119-155 */
+    LBM_TRT = 1  /* TRTCollision.hpp:146-191 */
+} lbm_collision;
+
+typedef enum lbm_open_policy {
+    LBM_ZOUHE_VELOCITY = 0, /* ZouHePolicy.hpp:58-123 */
+    LBM_ZOUHE_PRESSURE = 1, /* no reference counterpart: prescribed density */
+    LBM_OPEN_NOOP = 2       /* OpenBoundaryPolicy.hpp:58-61 */
+} lbm_open_policy;
+
+typedef enum lbm_storage {
+    LBM_STORE_F64 = 0,         /* plain FP64: the drop-in, parity path */
+    LBM_STORE_F32_SHIFTED = 1, /* f - w in FP32, FP32 collision, FP64 reductions */
+    LBM_STORE_F16_SHIFTED = 2  /* (f - w) * scale in FP16, FP32 collision, FP64 reductions */
+} lbm_storage;
+
+typedef enum lbm_step_flags {
+    LBM_STEP_MACRO = 1u,    /* the last step of the call also stores rho/u (reference global_rho/global_u) */
+    LBM_STEP_CHECK = 2u,    /* synchronise after the call and report LBM_ERR_NAN */
+    LBM_STEP_NO_GRAPH = 4u  /* launch every step eagerly instead of replaying a captured HIP graph */
+} lbm_step_flags;
+
+typedef struct lbm_desc {
+    int64_t n_rows;        /* NR (height), >= 3 */
+    int64_t n_cols;        /* NC (width),  >= 3 */
+    int32_t collision;     /* lbm_collision */
+    int32_t storage;       /* lbm_storage */
+    double tau;            /* BGK relaxation time (BGKCollisionPolicy<2>::tau, BGKCollision.hpp:67) */
+    double tau_even;       /* TRT tau+ (TRTCollision.hpp:69) */
+    double tau_odd;        /* TRT tau- (TRTCollision.hpp:70) */
+    int32_t inlet_policy;  /* lbm_open_policy of the inlet list  (LatticeConfiguration inlet_policy_t) */
+    int32_t outlet_policy; /* lbm_open_policy of the outlet list (LatticeConfiguration outlet_policy_t) */
+    double f16_scale;      /* FP16 storage scale (power of two); 0 = chosen from the boundary values */
+    int32_t device;        /* HIP device ordinal */
+    int32_t reserved;
+} lbm_desc;
+
+/* Library version string. */
+const char *lbm_version(void);
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char *lbm_last_error(void);
+
+/* Creates a context.  Replaces constructing Lattice<Policy> with its three
+ * population tensors (Lattice.hpp:173-205) and the static parameter setters
+ * BGKCollisionPolicy<2>::initialize (BGKCollision.hpp:75-88) and
+ * TRTCollisionPolicy<2>::initialize (TRTCollision.hpp:83-93). */
+int lbm_create(const lbm_desc *desc, lbm_ctx **out);
+
+/* Frees all device memory of the context. */
+int lbm_destroy(lbm_ctx *ctx);
+
+/* Changes the relaxation parameters (BGKCollisionPolicy<2>::initialize,
+ * TRTCollisionPolicy<2>::initialize / enforce_magic_parameter,
+ * TRTCollision.hpp:128-134). */
+int lbm_set_relaxation(lbm_ctx *ctx, double tau, double tau_even, double tau_odd);
+
+/* Installs the geometry from a class plane (ColMajor NR*NC).  Replaces
+ * Lattice::reinit (Lattice.hpp:453-481) with the node lists that
+ * build_lattice / read_lattice_file produce (ConstructionTools.hpp:54-201,
+ * LatticeReader.hpp:367-522), including identify_node_type and
+ * identify_obstacle_propagation (NodeIdentification.hpp:45-242).
+ * Inlet/outlet nodes must lie on the domain perimeter (the reference
+ * asserts it, LatticeReader.hpp:288-300); SOLID nodes next to a wet node
+ * are LBM_ERR_UNSUPPORTED. */
+int lbm_set_geometry(lbm_ctx *ctx, const uint8_t *node_class);
+
+/* Number of nodes per class after set_geometry: out[0..5] indexed by
+ * lbm_node_class, out[6] = wet obstacle nodes (directions mask != 0). */
+int lbm_node_counts(lbm_ctx *ctx, int64_t out[7]);
+
+/* Inlet (which = 0) or outlet (which = 1) nodes in list order -- the
+ * row-major order of the reference's lists -- as (i, j) pairs; *count
+ * receives the number of nodes (ij may be NULL to query the count). */
+int lbm_open_nodes(lbm_ctx *ctx, int which, int64_t *ij, int64_t *count);
+
+/* Prescribed values of the inlet/outlet nodes for the coming step, in
+ * list order.  ux/uy are the velocity the reference's update functions
+ * return (VelocityInitializer<2>::update_nodes,
+ * InletOutletInitializer.hpp:90-129); rho is the density of a
+ * LBM_ZOUHE_PRESSURE list.  NULL leaves a component unchanged.  Values
+ * persist until set again. */
+int lbm_set_open_values(lbm_ctx *ctx, int which, const double *ux, const double *uy, const double *rho);
+
+/* Start of Lattice::perform_lbm (Lattice.hpp:294-328): rho = 1, u = 0 and
+ * populations = feq(1, 0) on the wet nodes; the step counter restarts at 0. */
+int lbm_init_equilibrium(lbm_ctx *ctx);
+
+/* Replaces the population state (NR*NC*9, reference layout).  Only wet
+ * nodes are read.  Does not change the step counter. */
+int lbm_upload_populations(lbm_ctx *ctx, const double *populations);
+
+/* Current populations in the reference layout -- the tensor
+ * Lattice::get_populations returns after the completed steps
+ * (Lattice.hpp:492).  Non-wet nodes read 0 (the reference holds
+ * order-dependent scratch there). */
+int lbm_download_populations(lbm_ctx *ctx, double *populations);
+
+/* Post-collision populations of the last completed step (the
+ * after_collision_populations tensor, Lattice.hpp:136-145); non-wet nodes
+ * read 0.  LBM_ERR_STATE before the first step. */
+int lbm_download_after_collision(lbm_ctx *ctx, double *populations);
+
+/* Advances n time steps.  Replaces the loop body of Lattice::perform_lbm
+ * (Lattice.hpp:338-423): update_nodes, update_macro, calc_equilibrium,
+ * collide, stream, wall/obstacle bounce-back, inlet/outlet boundaries --
+ * fused into one kernel per step.  flags: lbm_step_flags. */
+int lbm_step(lbm_ctx *ctx, int64_t n, uint32_t flags);
+
+/* Number of completed steps. */
+int lbm_get_time(lbm_ctx *ctx, int64_t *t);
+
+/* global_rho / global_u as the reference holds them after the last step
+ * called with LBM_STEP_MACRO (Lattice::get_global_rho/get_global_u,
+ * Lattice.hpp:499,506): moments of the pre-collision populations on fluid
+ * nodes, rho = 1 and the prescribed u on velocity-prescribed inlet/outlet
+ * nodes, rho = 1, u = 0 elsewhere. */
+int lbm_download_macro(lbm_ctx *ctx, double *rho, double *u);
+
+/* FlowAnalyzer<2>::compute_flow_properties (FlowAnalyzer.hpp:348-451) on
+ * the post-collision populations of the last step, over n points given as
+ * (i, j) pairs with their 9-bit direction masks (NULL, NULL = every wet
+ * obstacle node, in list order).  lift_drag[0] = total lift,
+ * lift_drag[1] = total drag, reduced in FP64 on the GPU.  per_point
+ * (optional, 2*n) receives (lift, drag) of each point. */
+int lbm_lift_drag(lbm_ctx *ctx, const int64_t *ij, const int32_t *masks, int64_t n, double lift_drag[2],
+                  double *per_point);
+
+/* FP64 reductions over the current populations: out[0] = mass of the wet
+ * nodes, out[1] = x momentum and out[2] = y momentum of the fluid nodes,
+ * out[3] = mass of the fluid nodes, out[4] = number of fluid nodes.  No
+ * reference counterpart (the reference has no reductions besides
+ * FlowAnalyzer); used for mass drift and Darcy permeability. */
+int lbm_reduce(lbm_ctx *ctx, double out[5]);
+
+/* Waits for all work queued on the context. */
+int lbm_synchronize(lbm_ctx *ctx);
+
+/* The HIP stream the context queues its work on (hipStream_t). */
+int lbm_get_stream(lbm_ctx *ctx, void **stream);
+
+/* Storage scale in use (1 unless LBM_STORE_F16_SHIFTED). */
+int lbm_get_scale(lbm_ctx *ctx, double *scale);
+
+/* ------------------------------------------------------------------ */
+/* Geometry helpers (host only; write a class plane, ColMajor NR*NC)   */
+/* ------------------------------------------------------------------ */
+
+/* Lid-driven cavity: BOUNDARY perimeter, INLET on row 0, columns
+ * 1..NC-2.  Same node sets as ConstructionInfo::add_perimeter_nodes +
+ * add_nodes_interval({0,1},{0,NC-2},INLET) (ConstructionInfo.hpp:147-281)
+ * and as test/assets/lid-lattice.txt. */
+int lbm_geometry_cavity(int64_t nr, int64_t nc, uint8_t *node_class);
+
+/* Channel: BOUNDARY rows 0 and NR-1, INLET column 0 and OUTLET column
+ * NC-1 on rows 1..NR-2 (examples/3_Vortex.cpp:36-43). */
+int lbm_geometry_channel(int64_t nr, int64_t nc, uint8_t *node_class);
+
+/* Marks a disc of OBSTACLE nodes: integer (i-ci)^2 + (j-cj)^2 <= r^2, as
+ * ConstructionInfo::add_obstacle_hyper_sphere (ConstructionInfo.hpp:374-451);
+ * fails with LBM_ERR_INVALID, like the reference, when the disc would
+ * leave rows/columns 1..N-2.  *added (optional) = nodes newly marked. */
+int lbm_geometry_add_disc(int64_t nr, int64_t nc, uint8_t *node_class, int64_t ci, int64_t cj, int64_t r,
+                          int64_t *added);
+
+/* Random porous medium (no reference counterpart; the reference's
+ * add_obstacle_random_spheres draws a fixed count with libstdc++'s
+ * distribution, ConstructionInfo.hpp:462-486): discs with integer radius
+ * uniform in [rmin, rmax] and centres uniform over the domain, drawn from
+ * mt19937_64(seed) with the mapping lo + floor(x * (hi-lo+1) / 2^64);
+ * a disc leaving rows/columns 1..N-2 is redrawn; stops when the OBSTACLE
+ * fraction of all nodes reaches target_solid.  *solid receives the
+ * fraction reached, *ndiscs the number of discs placed. */
+int lbm_geometry_porous(int64_t nr, int64_t nc, uint8_t *node_class, uint64_t seed, int64_t rmin, int64_t rmax,
+                        double target_solid, double *solid, int64_t *ndiscs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBM_AMD_H */

@@ -1,0 +1,636 @@
+// C ABI of the D2Q9 engine (declared in include/lbm_amd.h).
+//
+// The context owns all device memory: two population buffers (A-B), the
+// bit-packed wet mask, the perimeter edge table (inlet/outlet kind,
+// prescribed values, frozen populations), the rho/u fields and reduction
+// scratch.  State machine:
+//   kNone  -- no populations yet
+//   kP     -- buffer `cur` holds pre-collision populations P_t (after
+//             init or upload); the next step starts with a collide-only pass
+//   kFstar -- buffer `cur` holds F*_{t-1}; the next step gathers from it
+// t counts completed reference time steps.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lbm_amd.h"
+#include "lbm_internal.h"
+
+using namespace lbm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                   \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess) return fail(LBM_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+enum State { kNone = 0, kP = 1, kFstar = 2 };
+
+}  // namespace
+
+struct lbm_ctx {
+    lbm_desc desc{};
+    int32_t storage = kF64;
+    size_t esize = 8;
+    hipStream_t stream = nullptr;
+    // geometry
+    bool has_geom = false;
+    int64_t nr = 0, nc = 0, ldr = 0, plane = 0, wl = 0, E = 0, segs = 0;
+    uint64_t *d_wet = nullptr;
+    uint8_t *d_edge_kind = nullptr;
+    void *d_frozen = nullptr;
+    void *d_F[2] = {nullptr, nullptr};
+    int cur = 0;
+    // prescribed values: two device copies [3][E]; `bc_prev` is the one the
+    // last step used as "current", `bc_cur` the one the next step uses.
+    double *d_bc[2] = {nullptr, nullptr};
+    double *h_bc_pin[2] = {nullptr, nullptr};
+    hipEvent_t bc_ev[2] = {nullptr, nullptr};
+    int bc_cur = 0, bc_prev = 0;
+    std::vector<double> h_bc;  // [3][E] latest values
+    double *d_macro = nullptr;
+    int32_t *d_flag = nullptr;
+    double *d_partials = nullptr;
+    int64_t partials_cap = 0;
+    double *d_out = nullptr;  // 8 doubles
+    int64_t *d_pts = nullptr;
+    int32_t *d_masks = nullptr;
+    double *d_pp = nullptr;
+    int64_t pts_cap = 0;
+    double *d_staging = nullptr;
+    // host-side geometry
+    std::vector<int64_t> open_ij[2], open_e[2];
+    std::vector<int64_t> obst_ij;
+    std::vector<int32_t> obst_mask;
+    int64_t counts[7] = {0, 0, 0, 0, 0, 0, 0};
+    int64_t n_wet = 0;
+    // dynamic state
+    int state = kNone;
+    int64_t t = 0;
+    float scale = 1.f;
+    bool macro_valid = false;
+};
+
+namespace {
+
+StepArgs make_args(const lbm_ctx *c)
+{
+    StepArgs a{};
+    a.g.nr = (int32_t)c->nr;
+    a.g.nc = (int32_t)c->nc;
+    a.g.ldr = c->ldr;
+    a.g.plane = c->plane;
+    a.g.wet = c->d_wet;
+    a.g.wl = c->wl;
+    a.g.E = c->E;
+    a.g.edge_kind = c->d_edge_kind;
+    a.frozen = c->d_frozen;
+    a.bc_prev = c->d_bc[c->bc_prev];
+    a.bc_cur = c->d_bc[c->bc_cur];
+    a.inlet_policy = c->desc.inlet_policy;
+    a.outlet_policy = c->desc.outlet_policy;
+    a.collision = c->desc.collision;
+    a.segs = (int32_t)c->segs;
+    // The reference recomputes t = 1.0 / tau inside every call
+    // (BGKCollision.hpp:140-141) and (1.0/tau_even) etc. inside every
+    // expression (TRTCollision.hpp:155-188); IEEE division makes these the
+    // same values when formed once here.
+    a.bgk_t = 1.0 / c->desc.tau;
+    a.bgk_omt = 1.0 - a.bgk_t;
+    a.trt_inv_te = 1.0 / c->desc.tau_even;
+    a.trt_a = 0.5 * (1.0 / c->desc.tau_even);
+    a.trt_b = 0.5 * (1.0 / c->desc.tau_odd);
+    a.scale = c->scale;
+    a.inv_scale = 1.f / c->scale;
+    a.flag = c->d_flag;
+    return a;
+}
+
+void free_geometry(lbm_ctx *c)
+{
+    (void)hipFree(c->d_wet); c->d_wet = nullptr;
+    (void)hipFree(c->d_edge_kind); c->d_edge_kind = nullptr;
+    (void)hipFree(c->d_frozen); c->d_frozen = nullptr;
+    for (int k = 0; k < 2; ++k) {
+        (void)hipFree(c->d_F[k]); c->d_F[k] = nullptr;
+        (void)hipFree(c->d_bc[k]); c->d_bc[k] = nullptr;
+        if (c->h_bc_pin[k]) (void)hipHostFree(c->h_bc_pin[k]);
+        c->h_bc_pin[k] = nullptr;
+    }
+    (void)hipFree(c->d_macro); c->d_macro = nullptr;
+    (void)hipFree(c->d_partials); c->d_partials = nullptr; c->partials_cap = 0;
+    (void)hipFree(c->d_pts); c->d_pts = nullptr;
+    (void)hipFree(c->d_masks); c->d_masks = nullptr;
+    (void)hipFree(c->d_pp); c->d_pp = nullptr; c->pts_cap = 0;
+    (void)hipFree(c->d_staging); c->d_staging = nullptr;
+    c->has_geom = false;
+    c->state = kNone;
+}
+
+int ensure_partials(lbm_ctx *c, int64_t n)
+{
+    if (n <= c->partials_cap) return LBM_OK;
+    (void)hipFree(c->d_partials);
+    c->d_partials = nullptr;
+    HIP_TRY(hipMalloc(&c->d_partials, (size_t)n * sizeof(double)));
+    c->partials_cap = n;
+    return LBM_OK;
+}
+
+int ensure_staging(lbm_ctx *c)
+{
+    if (c->d_staging) return LBM_OK;
+    HIP_TRY(hipMalloc(&c->d_staging, (size_t)(9 * c->nr * c->nc) * sizeof(double)));
+    return LBM_OK;
+}
+
+// Pushes the host values h_bc into the device copy the next step reads as
+// "current".  The copy that the last step read as "current" becomes
+// "previous" and stays untouched.
+int push_bc(lbm_ctx *c)
+{
+    if (c->bc_cur == c->bc_prev) c->bc_cur = 1 - c->bc_prev;
+    int k = c->bc_cur;
+    // The pinned staging copy k may still be in flight from an earlier push.
+    HIP_TRY(hipEventSynchronize(c->bc_ev[k]));
+    std::memcpy(c->h_bc_pin[k], c->h_bc.data(), c->h_bc.size() * sizeof(double));
+    HIP_TRY(hipMemcpyAsync(c->d_bc[k], c->h_bc_pin[k], c->h_bc.size() * sizeof(double), hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipEventRecord(c->bc_ev[k], c->stream));
+    return LBM_OK;
+}
+
+// Storage scale of the FP16 path: a power of two that puts the largest
+// expected |f - w| (from the prescribed velocities and densities) near
+// 2^11, far from both the FP16 overflow (65504) and the subnormal range.
+float choose_scale(const lbm_ctx *c)
+{
+    if (c->storage != kF16) return 1.f;
+    if (c->desc.f16_scale > 0) return (float)c->desc.f16_scale;
+    double umax = 0.0, dmax = 0.0;
+    for (int w = 0; w < 2; ++w)
+        for (int64_t e : c->open_e[w]) {
+            umax = std::max(umax, std::hypot(c->h_bc[e], c->h_bc[c->E + e]));
+            dmax = std::max(dmax, std::fabs(c->h_bc[2 * c->E + e] - 1.0));
+        }
+    double gmax = (4.0 / 9.0) * (dmax + 3.0 * umax + 6.0 * umax * umax);
+    gmax = std::max(gmax, 1e-4);
+    double s = std::exp2(std::floor(std::log2(2048.0 / gmax)));
+    s = std::min(std::max(s, 1.0), 16777216.0);
+    return (float)s;
+}
+
+bool on_perimeter(int64_t i, int64_t j, int64_t nr, int64_t nc)
+{
+    return i == 0 || j == 0 || i == nr - 1 || j == nc - 1;
+}
+
+int64_t edge_slot_host(int64_t i, int64_t j, int64_t nr, int64_t nc)
+{
+    if (i == 0) return j;
+    if (i == nr - 1) return nc + j;
+    if (j == 0) return 2 * nc + i;
+    return 2 * nc + nr + i;
+}
+
+const int kOffI[9] = {0, 0, -1, 0, 1, -1, -1, 1, 1};
+const int kOffJ[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+
+}  // namespace
+
+extern "C" {
+
+const char *lbm_version(void) { return "lbm_amd 0.1 (gfx950)"; }
+
+const char *lbm_last_error(void) { return g_err.c_str(); }
+
+int lbm_create(const lbm_desc *desc, lbm_ctx **out)
+{
+    if (!desc || !out) return fail(LBM_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (desc->n_rows < 3 || desc->n_cols < 3) return fail(LBM_ERR_INVALID, "lattice must be at least 3x3");
+    if (desc->n_rows > (1 << 30) || desc->n_cols > (1 << 30)) return fail(LBM_ERR_INVALID, "lattice too large");
+    if (desc->collision != LBM_BGK && desc->collision != LBM_TRT) return fail(LBM_ERR_INVALID, "unknown collision");
+    if (desc->storage < LBM_STORE_F64 || desc->storage > LBM_STORE_F16_SHIFTED)
+        return fail(LBM_ERR_INVALID, "unknown storage");
+    for (int32_t p : {desc->inlet_policy, desc->outlet_policy})
+        if (p < LBM_ZOUHE_VELOCITY || p > LBM_OPEN_NOOP) return fail(LBM_ERR_INVALID, "unknown open policy");
+    if (desc->collision == LBM_BGK && !(desc->tau > 0)) return fail(LBM_ERR_INVALID, "tau must be > 0");
+    if (desc->collision == LBM_TRT && !(desc->tau_even > 0 && desc->tau_odd > 0))
+        return fail(LBM_ERR_INVALID, "tau_even and tau_odd must be > 0");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (desc->device < 0 || desc->device >= ndev) return fail(LBM_ERR_INVALID, "no HIP device %d", desc->device);
+    HIP_TRY(hipSetDevice(desc->device));
+    lbm_ctx *c = new lbm_ctx();
+    c->desc = *desc;
+    c->storage = desc->storage;
+    c->esize = desc->storage == kF64 ? 8 : desc->storage == kF32 ? 4 : 2;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(LBM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    for (int k = 0; k < 2; ++k) {
+        e = hipEventCreateWithFlags(&c->bc_ev[k], hipEventDisableTiming);
+        if (e != hipSuccess) {
+            lbm_destroy(c);
+            return fail(LBM_ERR_HIP, "hipEventCreate: %s", hipGetErrorString(e));
+        }
+    }
+    e = hipMalloc(&c->d_flag, 64);
+    if (e == hipSuccess) e = hipMalloc(&c->d_out, 8 * sizeof(double));
+    if (e == hipSuccess) e = hipMemsetAsync(c->d_flag, 0, 64, c->stream);
+    if (e != hipSuccess) {
+        lbm_destroy(c);
+        return fail(LBM_ERR_HIP, "allocation: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return LBM_OK;
+}
+
+int lbm_destroy(lbm_ctx *c)
+{
+    if (!c) return LBM_OK;
+    (void)hipSetDevice(c->desc.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_geometry(c);
+    (void)hipFree(c->d_flag);
+    (void)hipFree(c->d_out);
+    for (int k = 0; k < 2; ++k)
+        if (c->bc_ev[k]) (void)hipEventDestroy(c->bc_ev[k]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return LBM_OK;
+}
+
+int lbm_set_relaxation(lbm_ctx *c, double tau, double tau_even, double tau_odd)
+{
+    if (!c) return fail(LBM_ERR_INVALID, "null context");
+    if (c->desc.collision == LBM_BGK && !(tau > 0)) return fail(LBM_ERR_INVALID, "tau must be > 0");
+    if (c->desc.collision == LBM_TRT && !(tau_even > 0 && tau_odd > 0))
+        return fail(LBM_ERR_INVALID, "tau_even and tau_odd must be > 0");
+    c->desc.tau = tau;
+    c->desc.tau_even = tau_even;
+    c->desc.tau_odd = tau_odd;
+    return LBM_OK;
+}
+
+int lbm_set_geometry(lbm_ctx *c, const uint8_t *cls)
+{
+    if (!c || !cls) return fail(LBM_ERR_INVALID, "null argument");
+    HIP_TRY(hipSetDevice(c->desc.device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const int64_t nr = c->desc.n_rows, nc = c->desc.n_cols;
+    // validation
+    for (int64_t q = 0; q < nr * nc; ++q)
+        if (cls[q] > LBM_OBSTACLE) return fail(LBM_ERR_INVALID, "node class %d at %lld out of range", cls[q], (long long)q);
+    auto wet_cls = [](uint8_t k) { return k == LBM_FLUID || k == LBM_INLET || k == LBM_OUTLET; };
+    for (int64_t j = 0; j < nc; ++j)
+        for (int64_t i = 0; i < nr; ++i) {
+            uint8_t k = cls[i + nr * j];
+            if ((k == LBM_INLET || k == LBM_OUTLET) && !on_perimeter(i, j, nr, nc))
+                return fail(LBM_ERR_INVALID, "inlet/outlet node (%lld, %lld) is not on the domain perimeter",
+                            (long long)i, (long long)j);
+            if (k == LBM_SOLID) {
+                for (int d = 1; d < 9; ++d) {
+                    int64_t ii = i + kOffI[d], jj = j + kOffJ[d];
+                    if (ii >= 0 && ii < nr && jj >= 0 && jj < nc && wet_cls(cls[ii + nr * jj]))
+                        return fail(LBM_ERR_UNSUPPORTED,
+                                    "SOLID node (%lld, %lld) borders a wet node: unsupported", (long long)i,
+                                    (long long)j);
+                }
+            }
+        }
+    free_geometry(c);
+    c->nr = nr;
+    c->nc = nc;
+    const int64_t nrw = (nr + 63) / 64;
+    c->ldr = nrw * 64;
+    c->plane = ((c->ldr * nc + 63) / 64) * 64;
+    c->wl = nrw + 2;
+    c->E = 2 * nc + 2 * nr;
+    c->segs = (c->ldr + kBlockRows - 1) / kBlockRows;
+    // host tables
+    std::vector<uint64_t> wet((size_t)((nc + 2) * c->wl), 0);
+    std::vector<uint8_t> edge((size_t)c->E, kEdgePlain);
+    for (int k = 0; k < 7; ++k) c->counts[k] = 0;
+    c->n_wet = 0;
+    for (int w = 0; w < 2; ++w) { c->open_ij[w].clear(); c->open_e[w].clear(); }
+    c->obst_ij.clear();
+    c->obst_mask.clear();
+    for (int64_t j = 0; j < nc; ++j)
+        for (int64_t i = 0; i < nr; ++i) {
+            uint8_t k = cls[i + nr * j];
+            c->counts[k]++;
+            if (wet_cls(k)) {
+                wet[(size_t)((j + 1) * c->wl + 1 + i / 64)] |= 1ull << (i % 64);
+                c->n_wet++;
+            }
+            if (k == LBM_INLET || k == LBM_OUTLET) edge[(size_t)edge_slot_host(i, j, nr, nc)] = k == LBM_INLET ? kEdgeInlet : kEdgeOutlet;
+        }
+    // list order of the reference: row-major scan
+    for (int64_t i = 0; i < nr; ++i)
+        for (int64_t j = 0; j < nc; ++j) {
+            uint8_t k = cls[i + nr * j];
+            if (k == LBM_INLET || k == LBM_OUTLET) {
+                int w = k == LBM_INLET ? 0 : 1;
+                c->open_ij[w].push_back(i);
+                c->open_ij[w].push_back(j);
+                c->open_e[w].push_back(edge_slot_host(i, j, nr, nc));
+            } else if (k == LBM_OBSTACLE) {
+                int32_t m = 0;
+                for (int d = 1; d < 9; ++d) {
+                    int64_t ii = i + kOffI[d], jj = j + kOffJ[d];
+                    if (ii >= 0 && ii < nr && jj >= 0 && jj < nc && wet_cls(cls[ii + nr * jj])) m |= 1 << d;
+                }
+                if (m) {
+                    c->obst_ij.push_back(i);
+                    c->obst_ij.push_back(j);
+                    c->obst_mask.push_back(m);
+                }
+            }
+        }
+    c->counts[6] = (int64_t)c->obst_mask.size();
+    c->h_bc.assign((size_t)(3 * c->E), 0.0);
+    for (int64_t e = 0; e < c->E; ++e) c->h_bc[2 * c->E + e] = 1.0;
+    // device allocations
+    size_t fbytes = (size_t)(9 * c->plane) * c->esize;
+    HIP_TRY(hipMalloc(&c->d_wet, wet.size() * sizeof(uint64_t)));
+    HIP_TRY(hipMemcpy(c->d_wet, wet.data(), wet.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&c->d_edge_kind, (size_t)c->E));
+    HIP_TRY(hipMemcpy(c->d_edge_kind, edge.data(), (size_t)c->E, hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&c->d_frozen, (size_t)(9 * c->E) * c->esize));
+    HIP_TRY(hipMemset(c->d_frozen, 0, (size_t)(9 * c->E) * c->esize));
+    for (int k = 0; k < 2; ++k) {
+        HIP_TRY(hipMalloc(&c->d_F[k], fbytes));
+        HIP_TRY(hipMemset(c->d_F[k], 0, fbytes));
+        HIP_TRY(hipMalloc(&c->d_bc[k], (size_t)(3 * c->E) * sizeof(double)));
+        HIP_TRY(hipHostMalloc((void **)&c->h_bc_pin[k], (size_t)(3 * c->E) * sizeof(double), hipHostMallocDefault));
+        HIP_TRY(hipMemcpy(c->d_bc[k], c->h_bc.data(), (size_t)(3 * c->E) * sizeof(double), hipMemcpyHostToDevice));
+    }
+    c->bc_cur = c->bc_prev = 0;
+    HIP_TRY(hipMalloc(&c->d_macro, (size_t)(3 * nr * nc) * sizeof(double)));
+    c->has_geom = true;
+    c->state = kNone;
+    c->t = 0;
+    c->cur = 0;
+    c->macro_valid = false;
+    return LBM_OK;
+}
+
+int lbm_node_counts(lbm_ctx *c, int64_t out[7])
+{
+    if (!c || !out) return fail(LBM_ERR_INVALID, "null argument");
+    if (!c->has_geom) return fail(LBM_ERR_STATE, "no geometry");
+    for (int k = 0; k < 7; ++k) out[k] = c->counts[k];
+    return LBM_OK;
+}
+
+int lbm_open_nodes(lbm_ctx *c, int which, int64_t *ij, int64_t *count)
+{
+    if (!c || !count || (which != 0 && which != 1)) return fail(LBM_ERR_INVALID, "bad argument");
+    if (!c->has_geom) return fail(LBM_ERR_STATE, "no geometry");
+    *count = (int64_t)c->open_e[which].size();
+    if (ij) std::copy(c->open_ij[which].begin(), c->open_ij[which].end(), ij);
+    return LBM_OK;
+}
+
+int lbm_set_open_values(lbm_ctx *c, int which, const double *ux, const double *uy, const double *rho)
+{
+    if (!c || (which != 0 && which != 1)) return fail(LBM_ERR_INVALID, "bad argument");
+    if (!c->has_geom) return fail(LBM_ERR_STATE, "no geometry");
+    HIP_TRY(hipSetDevice(c->desc.device));
+    const auto &es = c->open_e[which];
+    for (size_t q = 0; q < es.size(); ++q) {
+        if (ux) c->h_bc[es[q]] = ux[q];
+        if (uy) c->h_bc[c->E + es[q]] = uy[q];
+        if (rho) {
+            if (!(rho[q] > 0)) return fail(LBM_ERR_INVALID, "prescribed density must be > 0");
+            c->h_bc[2 * c->E + es[q]] = rho[q];
+        }
+    }
+    return push_bc(c);
+}
+
+int lbm_init_equilibrium(lbm_ctx *c)
+{
+    if (!c) return fail(LBM_ERR_INVALID, "null context");
+    if (!c->has_geom) return fail(LBM_ERR_STATE, "no geometry");
+    HIP_TRY(hipSetDevice(c->desc.device));
+    c->scale = choose_scale(c);
+    StepArgs a = make_args(c);
+    HIP_TRY(launch_init_eq(c->storage, a, c->d_F[c->cur], c->d_frozen, c->d_macro, c->stream));
+    HIP_TRY(hipMemsetAsync(c->d_flag, 0, 64, c->stream));
+    c->state = kP;
+    c->t = 0;
+    c->macro_valid = true;
+    return LBM_OK;
+}
+
+int lbm_upload_populations(lbm_ctx *c, const double *P)
+{
+    if (!c || !P) return fail(LBM_ERR_INVALID, "null argument");
+    if (!c->has_geom) return fail(LBM_ERR_STATE, "no geometry");
+    HIP_TRY(hipSetDevice(c->desc.device));
+    int r = ensure_staging(c);
+    if (r) return r;
+    if (c->state == kNone) c->scale = choose_scale(c);
+    HIP_TRY(hipMemcpyAsync(c->d_staging, P, (size_t)(9 * c->nr * c->nc) * sizeof(double), hipMemcpyHostToDevice,
+                           c->stream));
+    StepArgs a = make_args(c);
+    HIP_TRY(launch_upload(c->storage, a, c->d_staging, c->d_F[c->cur], c->d_frozen, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->state = kP;
+    return LBM_OK;
+}
+
+int lbm_download_populations(lbm_ctx *c, double *P)
+{
+    if (!c || !P) return fail(LBM_ERR_INVALID, "null argument");
+    if (c->state == kNone) return fail(LBM_ERR_STATE, "no populations (call lbm_init_equilibrium)");
+    HIP_TRY(hipSetDevice(c->desc.device));
+    int r = ensure_staging(c);
+    if (r) return r;
+    StepArgs a = make_args(c);
+    a.fin = c->d_F[c->cur];
+    HIP_TRY(launch_readout(c->storage, c->state == kFstar, a, c->d_staging, nullptr, c->stream));
+    HIP_TRY(hipMemcpyAsync(P, c->d_staging, (size_t)(9 * c->nr * c->nc) * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_download_after_collision(lbm_ctx *c, double *P)
+{
+    if (!c || !P) return fail(LBM_ERR_INVALID, "null argument");
+    if (c->state != kFstar) return fail(LBM_ERR_STATE, "no step completed since the last init/upload");
+    HIP_TRY(hipSetDevice(c->desc.device));
+    int r = ensure_staging(c);
+    if (r) return r;
+    StepArgs a = make_args(c);
+    a.fin = c->d_F[c->cur];
+    HIP_TRY(launch_readout(c->storage, false, a, c->d_staging, nullptr, c->stream));
+    HIP_TRY(hipMemcpyAsync(P, c->d_staging, (size_t)(9 * c->nr * c->nc) * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_step(lbm_ctx *c, int64_t n, uint32_t flags)
+{
+    if (!c) return fail(LBM_ERR_INVALID, "null context");
+    if (n < 0) return fail(LBM_ERR_INVALID, "negative step count");
+    if (c->state == kNone) return fail(LBM_ERR_STATE, "no populations (call lbm_init_equilibrium)");
+    HIP_TRY(hipSetDevice(c->desc.device));
+    for (int64_t s = 0; s < n; ++s) {
+        StepArgs a = make_args(c);
+        a.fin = c->d_F[c->cur];
+        a.fout = c->d_F[1 - c->cur];
+        a.macro = ((flags & LBM_STEP_MACRO) && s == n - 1) ? c->d_macro : nullptr;
+        HIP_TRY(launch_step(c->storage, c->state == kFstar, a, c->stream));
+        c->cur = 1 - c->cur;
+        c->state = kFstar;
+        c->bc_prev = c->bc_cur;
+        c->t++;
+    }
+    if (n > 0 && (flags & LBM_STEP_MACRO)) c->macro_valid = true;
+    if (flags & LBM_STEP_CHECK) {
+        int32_t flag = 0;
+        HIP_TRY(hipMemcpyAsync(&flag, c->d_flag, sizeof flag, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (flag) return fail(LBM_ERR_NAN, "NaN density at or before step %lld", (long long)c->t);
+    }
+    return LBM_OK;
+}
+
+int lbm_get_time(lbm_ctx *c, int64_t *t)
+{
+    if (!c || !t) return fail(LBM_ERR_INVALID, "null argument");
+    *t = c->t;
+    return LBM_OK;
+}
+
+int lbm_download_macro(lbm_ctx *c, double *rho, double *u)
+{
+    if (!c) return fail(LBM_ERR_INVALID, "null context");
+    if (!c->macro_valid) return fail(LBM_ERR_STATE, "no macro fields (step with LBM_STEP_MACRO)");
+    HIP_TRY(hipSetDevice(c->desc.device));
+    size_t N = (size_t)(c->nr * c->nc);
+    if (rho) HIP_TRY(hipMemcpyAsync(rho, c->d_macro, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (u) HIP_TRY(hipMemcpyAsync(u, c->d_macro + N, 2 * N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_lift_drag(lbm_ctx *c, const int64_t *ij, const int32_t *masks, int64_t n, double lift_drag[2],
+                  double *per_point)
+{
+    if (!c || !lift_drag) return fail(LBM_ERR_INVALID, "null argument");
+    if (c->state != kFstar) return fail(LBM_ERR_STATE, "no step completed since the last init/upload");
+    if ((ij == nullptr) != (masks == nullptr)) return fail(LBM_ERR_INVALID, "ij and masks go together");
+    HIP_TRY(hipSetDevice(c->desc.device));
+    const int64_t *hij = ij ? ij : c->obst_ij.data();
+    const int32_t *hm = ij ? masks : c->obst_mask.data();
+    if (!ij) n = (int64_t)c->obst_mask.size();
+    if (n < 0) return fail(LBM_ERR_INVALID, "negative point count");
+    for (int64_t q = 0; q < n; ++q) {
+        int64_t i = hij[2 * q], j = hij[2 * q + 1];
+        int32_t m = hm[q];
+        if (i < 0 || i >= c->nr || j < 0 || j >= c->nc) return fail(LBM_ERR_INVALID, "point %lld outside the lattice", (long long)q);
+        for (int d = 1; d < 9; ++d)
+            if ((m >> d) & 1) {
+                int64_t ii = i + kOffI[d], jj = j + kOffJ[d];
+                if (ii < 0 || ii >= c->nr || jj < 0 || jj >= c->nc)
+                    return fail(LBM_ERR_INVALID, "point %lld: direction %d leaves the lattice", (long long)q, d);
+            }
+    }
+    if (n == 0) {
+        lift_drag[0] = lift_drag[1] = 0.0;
+        return LBM_OK;
+    }
+    if (n > c->pts_cap) {
+        (void)hipFree(c->d_pts); (void)hipFree(c->d_masks); (void)hipFree(c->d_pp);
+        c->d_pts = nullptr; c->d_masks = nullptr; c->d_pp = nullptr; c->pts_cap = 0;
+        HIP_TRY(hipMalloc(&c->d_pts, (size_t)(2 * n) * sizeof(int64_t)));
+        HIP_TRY(hipMalloc(&c->d_masks, (size_t)n * sizeof(int32_t)));
+        HIP_TRY(hipMalloc(&c->d_pp, (size_t)(2 * n) * sizeof(double)));
+        c->pts_cap = n;
+    }
+    HIP_TRY(hipMemcpyAsync(c->d_pts, hij, (size_t)(2 * n) * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_masks, hm, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    int64_t nb = (n + kBlockRows - 1) / kBlockRows;
+    int r = ensure_partials(c, 2 * nb);
+    if (r) return r;
+    StepArgs a = make_args(c);
+    HIP_TRY(launch_lift_drag(c->storage, a, c->d_F[c->cur], c->d_pts, c->d_masks, n, per_point ? c->d_pp : nullptr,
+                             c->d_partials, &nb, c->stream));
+    HIP_TRY(launch_reduce_partials(c->d_partials, nb, 2, c->d_out, c->stream));
+    HIP_TRY(hipMemcpyAsync(lift_drag, c->d_out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (per_point)
+        HIP_TRY(hipMemcpyAsync(per_point, c->d_pp, (size_t)(2 * n) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_reduce(lbm_ctx *c, double out[5])
+{
+    if (!c || !out) return fail(LBM_ERR_INVALID, "null argument");
+    if (c->state == kNone) return fail(LBM_ERR_STATE, "no populations");
+    HIP_TRY(hipSetDevice(c->desc.device));
+    StepArgs a = make_args(c);
+    a.fin = c->d_F[c->cur];
+    int64_t nb = readout_blocks(a);
+    int r = ensure_partials(c, 5 * nb);
+    if (r) return r;
+    HIP_TRY(launch_readout(c->storage, c->state == kFstar, a, nullptr, c->d_partials, c->stream));
+    HIP_TRY(launch_reduce_partials(c->d_partials, nb, 5, c->d_out, c->stream));
+    HIP_TRY(hipMemcpyAsync(out, c->d_out, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_synchronize(lbm_ctx *c)
+{
+    if (!c) return fail(LBM_ERR_INVALID, "null context");
+    HIP_TRY(hipSetDevice(c->desc.device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_get_stream(lbm_ctx *c, void **stream)
+{
+    if (!c || !stream) return fail(LBM_ERR_INVALID, "null argument");
+    *stream = (void *)c->stream;
+    return LBM_OK;
+}
+
+int lbm_get_scale(lbm_ctx *c, double *scale)
+{
+    if (!c || !scale) return fail(LBM_ERR_INVALID, "null argument");
+    *scale = c->scale;
+    return LBM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,76 @@
+// Internal interface between the C ABI (lbm_capi.hip) and the gfx950
+// kernels (lbm_kernels.hip).  Not installed; the public interface is
+// include/lbm_amd.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace lbm {
+
+enum Storage : int32_t { kF64 = 0, kF32 = 1, kF16 = 2 };
+enum Policy : int32_t { kVelocity = 0, kPressure = 1, kNoop = 2 };
+enum EdgeKind : uint8_t { kEdgePlain = 0, kEdgeInlet = 1, kEdgeOutlet = 2 };
+
+// Rows handled by one work-group: four wave64s stacked along the
+// contiguous (row) axis of one column.
+constexpr int kBlockRows = 256;
+
+// Device-side description of the lattice.
+//  * Populations: 9 planes of `plane` elements; node (i, j) of plane d at
+//    d*plane + j*ldr + i, with ldr = NR rounded up to 64 (a wave covers one
+//    aligned 64-row run of a column).
+//  * Wet mask: one bit per node, 64-bit words along rows, one guard word
+//    above and below every column and one guard column on each side, all
+//    zero: word (j, k) at wet[(j+1)*wl + (k+1)], bit i%64 of k = i/64.
+//  * Edge slots: every perimeter node owns slot e (top row e = j, bottom
+//    row NC + j, left column 2NC + i, right column 2NC + NR + i); they hold
+//    the inlet/outlet kind, the prescribed values and the frozen
+//    populations whose source lies outside the domain.
+struct Geometry {
+    int32_t nr, nc;
+    int64_t ldr;
+    int64_t plane;
+    const uint64_t *wet;
+    int64_t wl;
+    int64_t E;
+    const uint8_t *edge_kind;
+};
+
+struct StepArgs {
+    Geometry g;
+    const void *fin;      // state read by the step (F* of the previous step, or P)
+    void *fout;           // F* written by the step
+    const void *frozen;   // [9][E] populations with out-of-domain source, storage type
+    const double *bc_prev;  // [3][E]: ux, uy, rho prescribed at the previous step
+    const double *bc_cur;   // [3][E]: values prescribed at this step
+    int32_t inlet_policy, outlet_policy;
+    int32_t collision;      // 0 BGK, 1 TRT
+    int32_t segs;           // row segments (of kBlockRows) per column
+    int64_t nblocks;
+    double bgk_t, bgk_omt;              // 1/tau, 1 - 1/tau
+    double trt_inv_te, trt_a, trt_b;    // 1/tau+, 0.5/tau+, 0.5/tau-
+    float scale, inv_scale;             // FP16 storage scale
+    double *macro;          // optional [3][NC][NR]: rho, ux, uy
+    int32_t *flag;          // bit 0: NaN density seen
+};
+
+// Kernel launchers (lbm_kernels.hip).  All are asynchronous on `s`.
+hipError_t launch_step(int32_t storage, bool gather, const StepArgs &a, hipStream_t s);
+// P (or F* when gather is false) of every node in the reference layout
+// (NR*NC*9 doubles, zeros on non-wet nodes); optional FP64 partial sums
+// per block (5 each) for lbm_reduce.
+hipError_t launch_readout(int32_t storage, bool gather, const StepArgs &a, double *out, double *partials,
+                          hipStream_t s);
+hipError_t launch_upload(int32_t storage, const StepArgs &a, const double *src, void *dst, void *frozen,
+                         hipStream_t s);
+hipError_t launch_init_eq(int32_t storage, const StepArgs &a, void *dst, void *frozen, double *macro,
+                          hipStream_t s);
+hipError_t launch_lift_drag(int32_t storage, const StepArgs &a, const void *fstar, const int64_t *ij,
+                            const int32_t *masks, int64_t n, double *per_point, double *partials,
+                            int64_t *nblocks_out, hipStream_t s);
+hipError_t launch_reduce_partials(const double *partials, int64_t nblocks, int32_t ncomp, double *out,
+                                  hipStream_t s);
+int64_t readout_blocks(const StepArgs &a);
+
+}  // namespace lbm

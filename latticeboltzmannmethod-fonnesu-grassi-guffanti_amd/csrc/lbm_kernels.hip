@@ -1,0 +1,897 @@
+// gfx950 kernels of the D2Q9 engine.
+//
+// One fused kernel per time step, pull scheme, A-B buffers holding the
+// post-collision populations F* of the previous step.  For a wet node x
+// and direction d with source s = x - c_d (c_d = the push offset of d in
+// the reference's stream, BGKCollision.hpp:179-187):
+//   s wet            -> P_d(x) = F*_d(s)                 (stream)
+//   s in the domain,
+//   not wet          -> P_d(x) = F*_opp(d)(x)            (half-way bounce-back,
+//                                                         BounceBackPolicy.hpp:68-187)
+//   s outside        -> P_d(x) = frozen value, or the Zou-He closure on a
+//                       velocity/pressure boundary node (ZouHePolicy.hpp:58-123)
+// then the moments, the equilibrium and the BGK/TRT collision of the same
+// node, written to F*(x).  Each wet slot has exactly one writer in the
+// reference's push order (its source node), so the gather reproduces the
+// push + bounce-back + Zou-He sequence of Lattice::perform_lbm
+// (Lattice.hpp:351-410) slot for slot.
+//
+// The FP64 path evaluates every expression in the reference's order with
+// no contraction (the library is built with -ffp-contract=off), so it
+// matches the Serial policy bit for bit.  The shifted paths store
+// g = f - w (FP32) or (f - w) * scale (FP16) and compute in FP32.
+//
+// Memory traffic per wet node and step: 9 loads + 9 stores of the storage
+// type plus the mask bits (read as wave-uniform words).  No LDS: every
+// source value is read by exactly one destination, so the only reuse is
+// the line shared by neighbouring waves of one column, which L2 holds.
+#include "lbm_internal.h"
+
+namespace lbm {
+
+namespace {
+
+// Push offsets of direction d (rows, columns), BGKCollision.hpp:179-187.
+__device__ constexpr int kOffI[9] = {0, 0, -1, 0, 1, -1, -1, 1, 1};
+__device__ constexpr int kOffJ[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+__device__ constexpr int kOpp[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+// Lattice weights (DefaultEquilibrium.hpp:68-76) and the physical
+// velocity of each direction: x along +column, y along -row.
+__device__ constexpr double kW[9] = {4.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0,
+                                      1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+__device__ constexpr int kCx[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+__device__ constexpr int kCy[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+
+// Boundary types of the reference's btypes enum (aliases.hpp:28-38).
+enum : int { BT_NONE = 0, BT_TOP, BT_RIGHT, BT_BOTTOM, BT_LEFT, BT_CORNER };
+
+// Spreads the blocks of a launch so that each XCD (blocks are dealt
+// round-robin to the 8 XCDs) gets one contiguous run of the column-major
+// tile order: neighbouring row segments of a column then share an L2.
+__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t nb)
+{
+    int64_t q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+__device__ __forceinline__ int64_t edge_slot(int i, int j, int nr, int nc)
+{
+    if (i == 0) return j;
+    if (i == nr - 1) return (int64_t)nc + j;
+    if (j == 0) return 2 * (int64_t)nc + i;
+    return 2 * (int64_t)nc + nr + i;
+}
+
+// NodeIdentification.hpp:45-110 for a perimeter node.
+__device__ __forceinline__ int edge_btype(int i, int j, int nr, int nc)
+{
+    bool top = i == 0, bottom = i == nr - 1, left = j == 0, right = j == nc - 1;
+    if ((top || bottom) && (left || right)) return BT_CORNER;
+    if (top) return BT_TOP;
+    if (bottom) return BT_BOTTOM;
+    if (left) return BT_LEFT;
+    if (right) return BT_RIGHT;
+    return BT_NONE;
+}
+
+// Node position of this thread: column j, row i, wave-uniform row base i0.
+struct Tile {
+    int i, j;
+    int64_t i0;
+    int lane;
+    bool valid;
+};
+
+__device__ __forceinline__ Tile tile_of(const StepArgs &a)
+{
+    Tile t;
+    int64_t b = xcd_remap(blockIdx.x, a.nblocks);
+    t.j = (int)(b / a.segs);
+    int64_t seg = b - (int64_t)t.j * a.segs;
+    int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    t.lane = threadIdx.x & 63;
+    t.i0 = seg * kBlockRows + wave * 64;
+    t.valid = t.i0 < a.g.ldr;
+    t.i = (int)t.i0 + t.lane;
+    return t;
+}
+
+// Wet bits of the 3x3 neighbourhood.  The three words of each of the three
+// columns are wave-uniform; bit (row) selects the lane.  Returns the mask
+// of directions whose source is wet in bits 1..8 and the node's own bit in
+// bit 0.
+__device__ __forceinline__ uint32_t neighbour_mask(const Geometry &g, const Tile &t)
+{
+    int64_t k = t.i0 >> 6;
+    const uint64_t *base = g.wet + (int64_t)t.j * g.wl + k;  // column j-1, word k-1
+    uint64_t up[3], mid[3], dn[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const uint64_t *col = base + c * g.wl;
+        uint64_t w0 = col[0], w1 = col[1], w2 = col[2];
+        up[c] = (w1 << 1) | (w0 >> 63);   // bit r = row r-1
+        mid[c] = w1;                      // bit r = row r
+        dn[c] = (w1 >> 1) | (w2 << 63);   // bit r = row r+1
+    }
+    int l = t.lane;
+    uint32_t m = (uint32_t)((mid[1] >> l) & 1);
+    m |= (uint32_t)((mid[0] >> l) & 1) << 1;  // d1: source (i, j-1)
+    m |= (uint32_t)((dn[1] >> l) & 1) << 2;   // d2: source (i+1, j)
+    m |= (uint32_t)((mid[2] >> l) & 1) << 3;  // d3: source (i, j+1)
+    m |= (uint32_t)((up[1] >> l) & 1) << 4;   // d4: source (i-1, j)
+    m |= (uint32_t)((dn[0] >> l) & 1) << 5;   // d5: source (i+1, j-1)
+    m |= (uint32_t)((dn[2] >> l) & 1) << 6;   // d6: source (i+1, j+1)
+    m |= (uint32_t)((up[2] >> l) & 1) << 7;   // d7: source (i-1, j+1)
+    m |= (uint32_t)((up[0] >> l) & 1) << 8;   // d8: source (i-1, j-1)
+    return m;
+}
+
+// Streams the 9 populations of an interior wet node: one load per
+// direction, its address chosen between the neighbour (stream) and the
+// node's own opposite slot (bounce-back).
+template <typename T>
+__device__ __forceinline__ void gather_interior(const T *__restrict__ F, const Geometry &g, int64_t n, uint32_t m,
+                                                T *p)
+{
+    const int64_t P = g.plane, L = g.ldr;
+    const int64_t src[9] = {0, n - L, n + 1, n + L, n - 1, n + 1 - L, n + 1 + L, n - 1 + L, n - 1 - L};
+    p[0] = F[n];
+#pragma unroll
+    for (int d = 1; d < 9; ++d) {
+        bool wet = (m >> d) & 1;
+        int64_t idx = wet ? d * P + src[d] : kOpp[d] * P + n;
+        p[d] = F[idx];
+    }
+}
+
+// Same for a perimeter node: sources outside the domain read the frozen
+// slot of the node's edge entry.
+template <typename T>
+__device__ __forceinline__ void gather_edge(const T *__restrict__ F, const T *__restrict__ frozen, const Geometry &g,
+                                            int i, int j, int64_t n, int64_t e, uint32_t m, T *p)
+{
+    p[0] = F[n];
+#pragma unroll
+    for (int d = 1; d < 9; ++d) {
+        int si = i - kOffI[d], sj = j - kOffJ[d];
+        bool out = si < 0 || si >= g.nr || sj < 0 || sj >= g.nc;
+        if (out) {
+            p[d] = frozen[d * g.E + e];
+        } else {
+            bool wet = (m >> d) & 1;
+            int64_t idx = wet ? d * g.plane + (int64_t)sj * g.ldr + si : kOpp[d] * g.plane + n;
+            p[d] = F[idx];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- FP64 --
+
+// DefaultEquilibrium.hpp:62-76, same expression order.
+__device__ __forceinline__ void feq64(double rho, double ux, double uy, double *e)
+{
+    const double two9 = (2.0 / 9.0);
+    const double one18 = (1.0 / 18.0);
+    const double one36 = (1.0 / 36.0);
+    double u2 = ux * ux + uy * uy;
+    e[0] = two9 * rho * (2 - 3 * u2);
+    e[1] = one18 * rho * (2 + 6 * ux + 9 * ux * ux - 3 * u2);
+    e[2] = one18 * rho * (2 + 6 * uy + 9 * uy * uy - 3 * u2);
+    e[3] = one18 * rho * (2 - 6 * ux + 9 * ux * ux - 3 * u2);
+    e[4] = one18 * rho * (2 - 6 * uy + 9 * uy * uy - 3 * u2);
+    e[5] = one36 * rho * (1 + 3 * (ux + uy) + 9 * ux * uy + 3 * u2);
+    e[6] = one36 * rho * (1 - 3 * (ux - uy) - 9 * ux * uy + 3 * u2);
+    e[7] = one36 * rho * (1 - 3 * (ux + uy) + 9 * ux * uy + 3 * u2);
+    e[8] = one36 * rho * (1 + 3 * (ux - uy) - 9 * ux * uy + 3 * u2);
+}
+
+// DefaultEquilibrium.hpp:145-153
+__device__ __forceinline__ void moments64(const double *p, double &rho, double &ux, double &uy)
+{
+    rho = p[0] + p[1] + p[2] + p[3] + p[4] + p[5] + p[6] + p[7] + p[8];
+    double rhoinv = 1.0 / rho;
+    ux = rhoinv * (p[1] + p[5] + p[8] - (p[3] + p[6] + p[7]));
+    uy = rhoinv * (p[2] + p[5] + p[6] - (p[4] + p[7] + p[8]));
+}
+
+// ZouHePolicy.hpp:70-118 (velocity prescribed).  The populations are
+// copied to scalars, the branch of the node's side rewrites its three
+// unknowns, and all nine are stored back unconditionally: stores into
+// different array elements on different branches would otherwise be merged
+// through a pointer phi and push the array into scratch memory.
+__device__ __forceinline__ void zouhe_velocity64(int bt, double *p, double ux, double uy)
+{
+    const double two_thirds = 2.0 / 3.0;
+    const double one_sixth = 1.0 / 6.0;
+    const double one_half = 0.5;
+    double p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3], p4 = p[4], p5 = p[5], p6 = p[6], p7 = p[7], p8 = p[8];
+    double rho, ru, rv;
+    if (bt == BT_TOP) {
+        rho = (p0 + p1 + p3 + 2.0 * (p2 + p5 + p6)) / (1.0 + uy);
+        ru = rho * ux; rv = rho * uy;
+        p4 = p2 - two_thirds * rv;
+        p7 = p5 - one_sixth * rv - one_half * ru + one_half * (p1 - p3);
+        p8 = p6 - one_sixth * rv + one_half * ru - one_half * (p1 - p3);
+    } else if (bt == BT_BOTTOM) {
+        rho = (p0 + p1 + p3 + 2.0 * (p4 + p7 + p8)) / (1.0 - uy);
+        ru = rho * ux; rv = rho * uy;
+        p2 = p4 + two_thirds * rv;
+        p5 = p7 + one_sixth * rv + one_half * ru - one_half * (p1 - p3);
+        p6 = p8 + one_sixth * rv - one_half * ru + one_half * (p1 - p3);
+    } else if (bt == BT_LEFT) {
+        rho = (p0 + p2 + p4 + 2.0 * (p3 + p7 + p6)) / (1.0 - ux);
+        ru = rho * ux; rv = rho * uy;
+        p1 = p3 + two_thirds * ru;
+        p5 = p7 + one_sixth * ru + one_half * rv - one_half * (p2 - p4);
+        p8 = p6 + one_sixth * ru - one_half * rv + one_half * (p2 - p4);
+    } else if (bt == BT_RIGHT) {
+        rho = (p0 + p2 + p4 + 2.0 * (p1 + p5 + p8)) / (1.0 + ux);
+        ru = rho * ux; rv = rho * uy;
+        p3 = p1 - two_thirds * ru;
+        p7 = p5 - one_sixth * ru - one_half * rv + one_half * (p2 - p4);
+        p6 = p8 - one_sixth * ru + one_half * rv - one_half * (p2 - p4);
+    }
+    // corners and NONE are left alone (ZouHePolicy.hpp:119-120)
+    p[1] = p1; p[2] = p2; p[3] = p3; p[4] = p4; p[5] = p5; p[6] = p6; p[7] = p7; p[8] = p8;
+}
+
+// Pressure closure: density prescribed, normal velocity from the known
+// populations, tangential velocity zero, unknowns from the velocity closure.
+__device__ __forceinline__ void zouhe_pressure64(int bt, double *p, double rho)
+{
+    double ux = 0.0, uy = 0.0;
+    if (bt == BT_TOP) uy = (p[0] + p[1] + p[3] + 2.0 * (p[2] + p[5] + p[6])) / rho - 1.0;
+    else if (bt == BT_BOTTOM) uy = 1.0 - (p[0] + p[1] + p[3] + 2.0 * (p[4] + p[7] + p[8])) / rho;
+    else if (bt == BT_LEFT) ux = 1.0 - (p[0] + p[2] + p[4] + 2.0 * (p[3] + p[7] + p[6])) / rho;
+    else if (bt == BT_RIGHT) ux = (p[0] + p[2] + p[4] + 2.0 * (p[1] + p[5] + p[8])) / rho - 1.0;
+    else return;
+    const double two_thirds = 2.0 / 3.0;
+    const double one_sixth = 1.0 / 6.0;
+    const double one_half = 0.5;
+    double p1 = p[1], p2 = p[2], p3 = p[3], p4 = p[4], p5 = p[5], p6 = p[6], p7 = p[7], p8 = p[8];
+    double ru = rho * ux, rv = rho * uy;
+    if (bt == BT_TOP) {
+        p4 = p2 - two_thirds * rv;
+        p7 = p5 - one_sixth * rv - one_half * ru + one_half * (p1 - p3);
+        p8 = p6 - one_sixth * rv + one_half * ru - one_half * (p1 - p3);
+    } else if (bt == BT_BOTTOM) {
+        p2 = p4 + two_thirds * rv;
+        p5 = p7 + one_sixth * rv + one_half * ru - one_half * (p1 - p3);
+        p6 = p8 + one_sixth * rv - one_half * ru + one_half * (p1 - p3);
+    } else if (bt == BT_LEFT) {
+        p1 = p3 + two_thirds * ru;
+        p5 = p7 + one_sixth * ru + one_half * rv - one_half * (p2 - p4);
+        p8 = p6 + one_sixth * ru - one_half * rv + one_half * (p2 - p4);
+    } else {
+        p3 = p1 - two_thirds * ru;
+        p7 = p5 - one_sixth * ru - one_half * rv + one_half * (p2 - p4);
+        p6 = p8 - one_sixth * ru + one_half * rv - one_half * (p2 - p4);
+    }
+    p[1] = p1; p[2] = p2; p[3] = p3; p[4] = p4; p[5] = p5; p[6] = p6; p[7] = p7; p[8] = p8;
+}
+
+// Pre-collision populations of wet node (i, j) in FP64, with boundary
+// handling.  Returns the open-boundary policy of the node (-1 = fluid).
+template <bool GATHER>
+__device__ __forceinline__ int load_p64(const StepArgs &a, const double *__restrict__ F, int i, int j, int64_t n,
+                                        uint32_t m, bool edge, int64_t &e, double *p)
+{
+    int policy = -1;
+    if (!edge) {
+        if (GATHER) gather_interior(F, a.g, n, m, p);
+        else
+#pragma unroll
+            for (int d = 0; d < 9; ++d) p[d] = F[d * a.g.plane + n];
+        return -1;
+    }
+    e = edge_slot(i, j, a.g.nr, a.g.nc);
+    if (GATHER) gather_edge(F, (const double *)a.frozen, a.g, i, j, n, e, m, p);
+    else
+#pragma unroll
+        for (int d = 0; d < 9; ++d) p[d] = F[d * a.g.plane + n];
+    int kind = a.g.edge_kind[e];
+    if (kind != kEdgePlain) {
+        policy = kind == kEdgeInlet ? a.inlet_policy : a.outlet_policy;
+        if (GATHER) {
+            int bt = edge_btype(i, j, a.g.nr, a.g.nc);
+            if (policy == kVelocity) zouhe_velocity64(bt, p, a.bc_prev[e], a.bc_prev[a.g.E + e]);
+            else if (policy == kPressure) zouhe_pressure64(bt, p, a.bc_prev[2 * a.g.E + e]);
+        }
+    }
+    return policy;
+}
+
+template <bool GATHER>
+__global__ __launch_bounds__(kBlockRows) void step_f64(StepArgs a)
+{
+    Tile t = tile_of(a);
+    if (!t.valid) return;
+    uint32_t m = neighbour_mask(a.g, t);
+    if (!(m & 1)) return;
+    const int i = t.i, j = t.j;
+    const int64_t n = (int64_t)j * a.g.ldr + i;
+    const bool edge = (i == 0) | (i == a.g.nr - 1) | (j == 0) | (j == a.g.nc - 1);
+    const double *__restrict__ F = (const double *)a.fin;
+    double p[9];
+    int64_t e = 0;
+    int policy = load_p64<GATHER>(a, F, i, j, n, m, edge, e, p);
+
+    // update_nodes + update_macro (Lattice.hpp:351-357): fluid and
+    // pressure nodes take their moments, velocity-prescribed nodes keep
+    // rho = 1 (Lattice.hpp:295; the Zou-He density write is lost on a
+    // by-value copy, ZouHePolicy.hpp:58,71) and the prescribed u.
+    double rho, ux, uy;
+    if (policy == kVelocity || policy == kNoop) {
+        rho = 1.0;
+        ux = a.bc_cur[e];
+        uy = a.bc_cur[a.g.E + e];
+        double s = p[0] + p[1] + p[2] + p[3] + p[4] + p[5] + p[6] + p[7] + p[8];
+        if (s != s) atomicOr(a.flag, 1);
+    } else {
+        moments64(p, rho, ux, uy);
+        if (rho != rho) atomicOr(a.flag, 1);
+    }
+    if (a.macro) {
+        int64_t q = (int64_t)j * a.g.nr + i, N = (int64_t)a.g.nr * a.g.nc;
+        a.macro[q] = rho;
+        a.macro[N + q] = ux;
+        a.macro[2 * N + q] = uy;
+    }
+    double eq[9], o[9];
+    feq64(rho, ux, uy, eq);
+    if (a.collision == 0) {
+        // BGKCollision.hpp:140-153
+        const double t_ = a.bgk_t, o_m_t = a.bgk_omt;
+#pragma unroll
+        for (int d = 0; d < 9; ++d) o[d] = o_m_t * p[d] + t_ * eq[d];
+    } else {
+        // TRTCollision.hpp:155-188
+        const double ca = a.trt_a, cb = a.trt_b;
+        o[0] = p[0] - a.trt_inv_te * (p[0] - eq[0]);
+#pragma unroll
+        for (int d = 1; d < 9; ++d) {
+            const int q = kOpp[d];
+            o[d] = p[d] - ca * (p[d] + p[q] - eq[d] - eq[q]) - cb * (p[d] - p[q] - eq[d] + eq[q]);
+        }
+    }
+    double *__restrict__ G = (double *)a.fout;
+#pragma unroll
+    for (int d = 0; d < 9; ++d) G[d * a.g.plane + n] = o[d];
+}
+
+// ------------------------------------------------------- shifted FP32 --
+
+template <typename T>
+__device__ __forceinline__ float to_g(T v, float inv_scale);
+template <>
+__device__ __forceinline__ float to_g<float>(float v, float) { return v; }
+template <>
+__device__ __forceinline__ float to_g<_Float16>(_Float16 v, float inv_scale) { return (float)v * inv_scale; }
+
+template <typename T>
+__device__ __forceinline__ T from_g(float g, float scale);
+template <>
+__device__ __forceinline__ float from_g<float>(float g, float) { return g; }
+template <>
+__device__ __forceinline__ _Float16 from_g<_Float16>(float g, float scale) { return (_Float16)(g * scale); }
+
+// Compensated running sum (Knuth two-sum; the error terms are added up
+// separately and folded in at the end).
+struct CSum {
+    float s = 0.f, c = 0.f;
+    __device__ __forceinline__ void add(float x)
+    {
+        float t = s + x;
+        float bp = t - s;
+        c += (s - (t - bp)) + (x - bp);
+        s = t;
+    }
+    __device__ __forceinline__ float value() const { return s + c; }
+};
+
+// Zou-He closures on shifted populations g = f - w.  The weights of the
+// known sets sum to 1 and opposite weights are equal, so the closures keep
+// their form with f -> g and rho -> 1 + (sum) as below.  Same scalar
+// structure as the FP64 closures.
+__device__ __forceinline__ void zouhe_velocity32(int bt, float *g, float ux, float uy)
+{
+    float g0 = g[0], g1 = g[1], g2 = g[2], g3 = g[3], g4 = g[4], g5 = g[5], g6 = g[6], g7 = g[7], g8 = g[8];
+    float rho, ru, rv;
+    if (bt == BT_TOP) {
+        rho = (1.f + (g0 + g1 + g3 + 2.f * (g2 + g5 + g6))) / (1.f + uy);
+        ru = rho * ux; rv = rho * uy;
+        g4 = g2 - (2.f / 3.f) * rv;
+        g7 = g5 - (1.f / 6.f) * rv - 0.5f * ru + 0.5f * (g1 - g3);
+        g8 = g6 - (1.f / 6.f) * rv + 0.5f * ru - 0.5f * (g1 - g3);
+    } else if (bt == BT_BOTTOM) {
+        rho = (1.f + (g0 + g1 + g3 + 2.f * (g4 + g7 + g8))) / (1.f - uy);
+        ru = rho * ux; rv = rho * uy;
+        g2 = g4 + (2.f / 3.f) * rv;
+        g5 = g7 + (1.f / 6.f) * rv + 0.5f * ru - 0.5f * (g1 - g3);
+        g6 = g8 + (1.f / 6.f) * rv - 0.5f * ru + 0.5f * (g1 - g3);
+    } else if (bt == BT_LEFT) {
+        rho = (1.f + (g0 + g2 + g4 + 2.f * (g3 + g7 + g6))) / (1.f - ux);
+        ru = rho * ux; rv = rho * uy;
+        g1 = g3 + (2.f / 3.f) * ru;
+        g5 = g7 + (1.f / 6.f) * ru + 0.5f * rv - 0.5f * (g2 - g4);
+        g8 = g6 + (1.f / 6.f) * ru - 0.5f * rv + 0.5f * (g2 - g4);
+    } else if (bt == BT_RIGHT) {
+        rho = (1.f + (g0 + g2 + g4 + 2.f * (g1 + g5 + g8))) / (1.f + ux);
+        ru = rho * ux; rv = rho * uy;
+        g3 = g1 - (2.f / 3.f) * ru;
+        g7 = g5 - (1.f / 6.f) * ru - 0.5f * rv + 0.5f * (g2 - g4);
+        g6 = g8 - (1.f / 6.f) * ru + 0.5f * rv - 0.5f * (g2 - g4);
+    }
+    g[1] = g1; g[2] = g2; g[3] = g3; g[4] = g4; g[5] = g5; g[6] = g6; g[7] = g7; g[8] = g8;
+}
+
+// drho0 = rho0 - 1 is formed in FP64 by the caller.
+__device__ __forceinline__ void zouhe_pressure32(int bt, float *g, float rho0, float drho0)
+{
+    float g0 = g[0], g1 = g[1], g2 = g[2], g3 = g[3], g4 = g[4], g5 = g[5], g6 = g[6], g7 = g[7], g8 = g[8];
+    float ux = 0.f, uy = 0.f;
+    if (bt == BT_TOP) uy = ((g0 + g1 + g3 + 2.f * (g2 + g5 + g6)) - drho0) / rho0;
+    else if (bt == BT_BOTTOM) uy = (drho0 - (g0 + g1 + g3 + 2.f * (g4 + g7 + g8))) / rho0;
+    else if (bt == BT_LEFT) ux = (drho0 - (g0 + g2 + g4 + 2.f * (g3 + g7 + g6))) / rho0;
+    else if (bt == BT_RIGHT) ux = ((g0 + g2 + g4 + 2.f * (g1 + g5 + g8)) - drho0) / rho0;
+    else return;
+    float ru = rho0 * ux, rv = rho0 * uy;
+    if (bt == BT_TOP) {
+        g4 = g2 - (2.f / 3.f) * rv;
+        g7 = g5 - (1.f / 6.f) * rv - 0.5f * ru + 0.5f * (g1 - g3);
+        g8 = g6 - (1.f / 6.f) * rv + 0.5f * ru - 0.5f * (g1 - g3);
+    } else if (bt == BT_BOTTOM) {
+        g2 = g4 + (2.f / 3.f) * rv;
+        g5 = g7 + (1.f / 6.f) * rv + 0.5f * ru - 0.5f * (g1 - g3);
+        g6 = g8 + (1.f / 6.f) * rv - 0.5f * ru + 0.5f * (g1 - g3);
+    } else if (bt == BT_LEFT) {
+        g1 = g3 + (2.f / 3.f) * ru;
+        g5 = g7 + (1.f / 6.f) * ru + 0.5f * rv - 0.5f * (g2 - g4);
+        g8 = g6 + (1.f / 6.f) * ru - 0.5f * rv + 0.5f * (g2 - g4);
+    } else {
+        g3 = g1 - (2.f / 3.f) * ru;
+        g7 = g5 - (1.f / 6.f) * ru - 0.5f * rv + 0.5f * (g2 - g4);
+        g6 = g8 - (1.f / 6.f) * ru + 0.5f * rv - 0.5f * (g2 - g4);
+    }
+    g[1] = g1; g[2] = g2; g[3] = g3; g[4] = g4; g[5] = g5; g[6] = g6; g[7] = g7; g[8] = g8;
+}
+
+template <typename T, bool GATHER>
+__device__ __forceinline__ int load_g(const StepArgs &a, const T *__restrict__ F, int i, int j, int64_t n, uint32_t m,
+                                      bool edge, int64_t &e, float *g)
+{
+    T raw[9];
+    int policy = -1;
+    if (!edge) {
+        if (GATHER) gather_interior(F, a.g, n, m, raw);
+        else
+#pragma unroll
+            for (int d = 0; d < 9; ++d) raw[d] = F[d * a.g.plane + n];
+#pragma unroll
+        for (int d = 0; d < 9; ++d) g[d] = to_g<T>(raw[d], a.inv_scale);
+        return -1;
+    }
+    e = edge_slot(i, j, a.g.nr, a.g.nc);
+    if (GATHER) gather_edge(F, (const T *)a.frozen, a.g, i, j, n, e, m, raw);
+    else
+#pragma unroll
+        for (int d = 0; d < 9; ++d) raw[d] = F[d * a.g.plane + n];
+#pragma unroll
+    for (int d = 0; d < 9; ++d) g[d] = to_g<T>(raw[d], a.inv_scale);
+    int kind = a.g.edge_kind[e];
+    if (kind != kEdgePlain) {
+        policy = kind == kEdgeInlet ? a.inlet_policy : a.outlet_policy;
+        if (GATHER) {
+            int bt = edge_btype(i, j, a.g.nr, a.g.nc);
+            if (policy == kVelocity) {
+                zouhe_velocity32(bt, g, (float)a.bc_prev[e], (float)a.bc_prev[a.g.E + e]);
+            } else if (policy == kPressure) {
+                double r0 = a.bc_prev[2 * a.g.E + e];
+                zouhe_pressure32(bt, g, (float)r0, (float)(r0 - 1.0));
+            }
+        }
+    }
+    return policy;
+}
+
+// Moments of shifted populations: drho = sum g, j = sum c g, compensated.
+__device__ __forceinline__ void moments32(const float *g, float &drho, float &jx, float &jy)
+{
+    CSum r, x, y;
+#pragma unroll
+    for (int d = 0; d < 9; ++d) r.add(g[d]);
+    x.add(g[1]); x.add(g[5]); x.add(g[8]); x.add(-g[3]); x.add(-g[6]); x.add(-g[7]);
+    y.add(g[2]); y.add(g[5]); y.add(g[6]); y.add(-g[4]); y.add(-g[7]); y.add(-g[8]);
+    drho = r.value();
+    jx = x.value();
+    jy = y.value();
+}
+
+template <typename T, bool GATHER>
+__global__ __launch_bounds__(kBlockRows) void step_shifted(StepArgs a)
+{
+    Tile t = tile_of(a);
+    if (!t.valid) return;
+    uint32_t m = neighbour_mask(a.g, t);
+    if (!(m & 1)) return;
+    const int i = t.i, j = t.j;
+    const int64_t n = (int64_t)j * a.g.ldr + i;
+    const bool edge = (i == 0) | (i == a.g.nr - 1) | (j == 0) | (j == a.g.nc - 1);
+    const T *__restrict__ F = (const T *)a.fin;
+    float g[9];
+    int64_t e = 0;
+    int policy = load_g<T, GATHER>(a, F, i, j, n, m, edge, e, g);
+
+    float drho, ux, uy, rho;
+    if (policy == kVelocity || policy == kNoop) {
+        drho = 0.f;
+        rho = 1.f;
+        ux = (float)a.bc_cur[e];
+        uy = (float)a.bc_cur[a.g.E + e];
+        float s = g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6] + g[7] + g[8];
+        if (s != s) atomicOr(a.flag, 1);
+    } else {
+        float jx, jy;
+        moments32(g, drho, jx, jy);
+        rho = 1.f + drho;
+        float rinv = 1.f / rho;
+        ux = jx * rinv;
+        uy = jy * rinv;
+        if (drho != drho) atomicOr(a.flag, 1);
+    }
+    if (a.macro) {
+        int64_t q = (int64_t)j * a.g.nr + i, N = (int64_t)a.g.nr * a.g.nc;
+        a.macro[q] = 1.0 + (double)drho;
+        a.macro[N + q] = ux;
+        a.macro[2 * N + q] = uy;
+    }
+    // Shifted equilibrium geq_d = feq_d - w_d
+    //   = w_d * (drho + rho * (3 cu + 4.5 cu^2 - 1.5 u^2)).
+    const float u15 = 1.5f * (ux * ux + uy * uy);
+    float geq[9];
+#pragma unroll
+    for (int d = 0; d < 9; ++d) {
+        float cu = (float)kCx[d] * ux + (float)kCy[d] * uy;
+        float poly = __builtin_fmaf(cu, __builtin_fmaf(4.5f, cu, 3.f), -u15);
+        geq[d] = (float)kW[d] * __builtin_fmaf(rho, poly, drho);
+    }
+    float o[9];
+    if (a.collision == 0) {
+        const float om = (float)a.bgk_t;
+#pragma unroll
+        for (int d = 0; d < 9; ++d) o[d] = __builtin_fmaf(om, geq[d] - g[d], g[d]);
+    } else {
+        const float ca = (float)a.trt_a, cb = (float)a.trt_b;
+        o[0] = __builtin_fmaf((float)a.trt_inv_te, geq[0] - g[0], g[0]);
+#pragma unroll
+        for (int d = 1; d < 9; ++d) {
+            const int q = kOpp[d];
+            float sym = (g[d] + g[q]) - (geq[d] + geq[q]);
+            float asym = (g[d] - g[q]) - (geq[d] - geq[q]);
+            o[d] = g[d] - __builtin_fmaf(ca, sym, cb * asym);
+        }
+    }
+    T *__restrict__ G = (T *)a.fout;
+#pragma unroll
+    for (int d = 0; d < 9; ++d) G[d * a.g.plane + n] = from_g<T>(o[d], a.scale);
+}
+
+// ------------------------------------------------- readout / reductions --
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// Sums NC values of every thread over the block; thread 0 writes them.
+template <int NC>
+__device__ __forceinline__ void block_sum(double *v, double *dst)
+{
+    __shared__ double red[kBlockRows / 64][NC];
+    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        double s = wave_sum(v[c]);
+        if (lane == 0) red[wave][c] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            double s = 0.0;
+            for (int w = 0; w < kBlockRows / 64; ++w) s += red[w][c];
+            dst[c] = s;
+        }
+    }
+}
+
+// Reads the pre-collision populations P (GATHER, from F*) or the stored
+// state as is (!GATHER) and writes them in FP64 in the reference layout.
+// Optional partial sums: wet mass, fluid momentum x/y, fluid mass, fluid
+// count.
+template <typename T, bool GATHER>
+__global__ __launch_bounds__(kBlockRows) void readout(StepArgs a, double *out, double *partials)
+{
+    Tile t = tile_of(a);
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (t.valid) {
+        uint32_t m = neighbour_mask(a.g, t);
+        const int i = t.i, j = t.j;
+        const int64_t N = (int64_t)a.g.nr * a.g.nc;
+        const int64_t q = (int64_t)j * a.g.nr + i;
+        if (m & 1) {
+            const int64_t n = (int64_t)j * a.g.ldr + i;
+            const bool edge = (i == 0) | (i == a.g.nr - 1) | (j == 0) | (j == a.g.nc - 1);
+            double f[9];
+            int64_t e = 0;
+            int policy;
+            if constexpr (sizeof(T) == 8) {
+                policy = load_p64<GATHER>(a, (const double *)a.fin, i, j, n, m, edge, e, f);
+            } else {
+                float g[9];
+                policy = load_g<T, GATHER>(a, (const T *)a.fin, i, j, n, m, edge, e, g);
+#pragma unroll
+                for (int d = 0; d < 9; ++d) f[d] = (double)g[d] + kW[d];
+            }
+            if (out) {
+#pragma unroll
+                for (int d = 0; d < 9; ++d) out[d * N + q] = f[d];
+            }
+            double rho = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
+            acc[0] = rho;
+            if (policy < 0) {
+                acc[1] = f[1] + f[5] + f[8] - (f[3] + f[6] + f[7]);
+                acc[2] = f[2] + f[5] + f[6] - (f[4] + f[7] + f[8]);
+                acc[3] = rho;
+                acc[4] = 1.0;
+            }
+        } else if (out && i < a.g.nr) {
+#pragma unroll
+            for (int d = 0; d < 9; ++d) out[d * N + q] = 0.0;
+        }
+    }
+    if (partials) block_sum<5>(acc, partials + (int64_t)blockIdx.x * 5);
+}
+
+// Converts a reference-layout FP64 state to the storage format on the wet
+// nodes and fills the frozen slots of the perimeter.
+template <typename T>
+__global__ __launch_bounds__(kBlockRows) void upload(StepArgs a, const double *src, T *dst, T *frozen)
+{
+    Tile t = tile_of(a);
+    if (!t.valid) return;
+    uint32_t m = neighbour_mask(a.g, t);
+    if (!(m & 1)) return;
+    const int i = t.i, j = t.j;
+    const int64_t N = (int64_t)a.g.nr * a.g.nc, q = (int64_t)j * a.g.nr + i;
+    const int64_t n = (int64_t)j * a.g.ldr + i;
+    const bool edge = (i == 0) | (i == a.g.nr - 1) | (j == 0) | (j == a.g.nc - 1);
+    const int64_t e = edge ? edge_slot(i, j, a.g.nr, a.g.nc) : 0;
+#pragma unroll
+    for (int d = 0; d < 9; ++d) {
+        double f = src[d * N + q];
+        T v;
+        if constexpr (sizeof(T) == 8) v = f;
+        else v = from_g<T>((float)(f - kW[d]), a.scale);
+        dst[d * a.g.plane + n] = v;
+        if (edge) frozen[d * a.g.E + e] = v;
+    }
+}
+
+// Lattice.hpp:294-328: feq(1, 0) = w_d on the wet nodes (0 in shifted
+// storage), rho = 1 and u = 0 everywhere.
+template <typename T>
+__global__ __launch_bounds__(kBlockRows) void init_eq(StepArgs a, T *dst, T *frozen, double *macro)
+{
+    Tile t = tile_of(a);
+    if (!t.valid) return;
+    const int i = t.i, j = t.j;
+    if (macro && i < a.g.nr) {
+        const int64_t N = (int64_t)a.g.nr * a.g.nc, q = (int64_t)j * a.g.nr + i;
+        macro[q] = 1.0;
+        macro[N + q] = 0.0;
+        macro[2 * N + q] = 0.0;
+    }
+    uint32_t m = neighbour_mask(a.g, t);
+    if (!(m & 1)) return;
+    const int64_t n = (int64_t)j * a.g.ldr + i;
+    const bool edge = (i == 0) | (i == a.g.nr - 1) | (j == 0) | (j == a.g.nc - 1);
+    const int64_t e = edge ? edge_slot(i, j, a.g.nr, a.g.nc) : 0;
+    double e9[9];
+    feq64(1.0, 0.0, 0.0, e9);
+#pragma unroll
+    for (int d = 0; d < 9; ++d) {
+        T v;
+        if constexpr (sizeof(T) == 8) v = e9[d];
+        else v = from_g<T>((float)(e9[d] - kW[d]), a.scale);
+        dst[d * a.g.plane + n] = v;
+        if (edge) frozen[d * a.g.E + e] = v;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ double fstar_at(const StepArgs &a, const T *F, int64_t i, int64_t j, int d)
+{
+    T v = F[d * a.g.plane + j * a.g.ldr + i];
+    if constexpr (sizeof(T) == 8) return (double)v;
+    else return (double)to_g<T>(v, a.inv_scale) + kW[d];
+}
+
+// FlowAnalyzer.hpp:386-430 for one point per thread, then a block sum.
+template <typename T>
+__global__ __launch_bounds__(kBlockRows) void lift_drag(StepArgs a, const T *F, const int64_t *ij,
+                                                        const int32_t *masks, int64_t n, double *per_point,
+                                                        double *partials)
+{
+    int64_t q = (int64_t)blockIdx.x * kBlockRows + threadIdx.x;
+    double acc[2] = {0.0, 0.0};
+    if (q < n) {
+        const int64_t i = ij[2 * q], j = ij[2 * q + 1];
+        const int32_t msk = masks[q];
+        const double h = sqrt(2.0);
+        double local_lift = 0., local_drag = 0.;
+        if (msk & (1 << 1)) local_drag -= fstar_at(a, F, i, j + 1, 3);
+        if (msk & (1 << 2)) local_lift -= fstar_at(a, F, i - 1, j, 4);
+        if (msk & (1 << 3)) local_drag += fstar_at(a, F, i, j - 1, 1);
+        if (msk & (1 << 4)) local_lift += fstar_at(a, F, i + 1, j, 2);
+        if (msk & (1 << 5)) {
+            double v = fstar_at(a, F, i - 1, j + 1, 7);
+            local_drag -= (v) * h / 2.0;
+            local_lift -= (v) * h / 2.0;
+        }
+        if (msk & (1 << 6)) {
+            double v = fstar_at(a, F, i - 1, j - 1, 8);
+            local_drag += (v) * h / 2.0;
+            local_lift -= (v) * h / 2.0;
+        }
+        if (msk & (1 << 7)) {
+            double v = fstar_at(a, F, i + 1, j - 1, 5);
+            local_drag += (v) * h / 2.0;
+            local_lift += (v) * h / 2.0;
+        }
+        if (msk & (1 << 8)) {
+            double v = fstar_at(a, F, i + 1, j + 1, 6);
+            local_drag -= (v) * h / 2.0;
+            local_lift += (v) * h / 2.0;
+        }
+        if (per_point) {
+            per_point[2 * q] = local_lift;
+            per_point[2 * q + 1] = local_drag;
+        }
+        acc[0] = local_lift;
+        acc[1] = local_drag;
+    }
+    block_sum<2>(acc, partials + (int64_t)blockIdx.x * 2);
+}
+
+// Second pass: one block folds the per-block partials in a fixed order.
+__global__ __launch_bounds__(kBlockRows) void reduce_partials(const double *partials, int64_t nb, int32_t ncomp,
+                                                              double *out)
+{
+    for (int c = 0; c < ncomp; ++c) {
+        double v = 0.0;
+        for (int64_t b = threadIdx.x; b < nb; b += kBlockRows) v += partials[b * ncomp + c];
+        double s[1] = {v};
+        double r[1];
+        block_sum<1>(s, r);
+        if (threadIdx.x == 0) out[c] = r[0];
+        __syncthreads();
+    }
+}
+
+int64_t blocks_of(const StepArgs &a) { return (int64_t)a.segs * a.g.nc; }
+
+template <typename K>
+hipError_t launch(K kernel, int64_t nb, hipStream_t s, StepArgs a)
+{
+    a.nblocks = nb;
+    hipLaunchKernelGGL(kernel, dim3((unsigned)nb), dim3(kBlockRows), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_step(int32_t storage, bool gather, const StepArgs &a, hipStream_t s)
+{
+    int64_t nb = blocks_of(a);
+    switch (storage) {
+    case kF64: return gather ? launch(step_f64<true>, nb, s, a) : launch(step_f64<false>, nb, s, a);
+    case kF32:
+        return gather ? launch(step_shifted<float, true>, nb, s, a) : launch(step_shifted<float, false>, nb, s, a);
+    default:
+        return gather ? launch(step_shifted<_Float16, true>, nb, s, a)
+                      : launch(step_shifted<_Float16, false>, nb, s, a);
+    }
+}
+
+int64_t readout_blocks(const StepArgs &a) { return blocks_of(a); }
+
+hipError_t launch_readout(int32_t storage, bool gather, const StepArgs &a0, double *out, double *partials,
+                          hipStream_t s)
+{
+    StepArgs a = a0;
+    a.nblocks = blocks_of(a);
+    dim3 grid((unsigned)a.nblocks), block(kBlockRows);
+    switch (storage) {
+    case kF64:
+        if (gather) hipLaunchKernelGGL((readout<double, true>), grid, block, 0, s, a, out, partials);
+        else hipLaunchKernelGGL((readout<double, false>), grid, block, 0, s, a, out, partials);
+        break;
+    case kF32:
+        if (gather) hipLaunchKernelGGL((readout<float, true>), grid, block, 0, s, a, out, partials);
+        else hipLaunchKernelGGL((readout<float, false>), grid, block, 0, s, a, out, partials);
+        break;
+    default:
+        if (gather) hipLaunchKernelGGL((readout<_Float16, true>), grid, block, 0, s, a, out, partials);
+        else hipLaunchKernelGGL((readout<_Float16, false>), grid, block, 0, s, a, out, partials);
+        break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_upload(int32_t storage, const StepArgs &a0, const double *src, void *dst, void *frozen,
+                         hipStream_t s)
+{
+    StepArgs a = a0;
+    a.nblocks = blocks_of(a);
+    dim3 grid((unsigned)a.nblocks), block(kBlockRows);
+    switch (storage) {
+    case kF64: hipLaunchKernelGGL(upload<double>, grid, block, 0, s, a, src, (double *)dst, (double *)frozen); break;
+    case kF32: hipLaunchKernelGGL(upload<float>, grid, block, 0, s, a, src, (float *)dst, (float *)frozen); break;
+    default:
+        hipLaunchKernelGGL(upload<_Float16>, grid, block, 0, s, a, src, (_Float16 *)dst, (_Float16 *)frozen);
+        break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_init_eq(int32_t storage, const StepArgs &a0, void *dst, void *frozen, double *macro,
+                          hipStream_t s)
+{
+    StepArgs a = a0;
+    a.nblocks = blocks_of(a);
+    dim3 grid((unsigned)a.nblocks), block(kBlockRows);
+    switch (storage) {
+    case kF64: hipLaunchKernelGGL(init_eq<double>, grid, block, 0, s, a, (double *)dst, (double *)frozen, macro); break;
+    case kF32: hipLaunchKernelGGL(init_eq<float>, grid, block, 0, s, a, (float *)dst, (float *)frozen, macro); break;
+    default:
+        hipLaunchKernelGGL(init_eq<_Float16>, grid, block, 0, s, a, (_Float16 *)dst, (_Float16 *)frozen, macro);
+        break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_lift_drag(int32_t storage, const StepArgs &a, const void *fstar, const int64_t *ij,
+                            const int32_t *masks, int64_t n, double *per_point, double *partials,
+                            int64_t *nblocks_out, hipStream_t s)
+{
+    int64_t nb = (n + kBlockRows - 1) / kBlockRows;
+    if (nb < 1) nb = 1;
+    *nblocks_out = nb;
+    dim3 grid((unsigned)nb), block(kBlockRows);
+    switch (storage) {
+    case kF64:
+        hipLaunchKernelGGL(lift_drag<double>, grid, block, 0, s, a, (const double *)fstar, ij, masks, n, per_point,
+                           partials);
+        break;
+    case kF32:
+        hipLaunchKernelGGL(lift_drag<float>, grid, block, 0, s, a, (const float *)fstar, ij, masks, n, per_point,
+                           partials);
+        break;
+    default:
+        hipLaunchKernelGGL(lift_drag<_Float16>, grid, block, 0, s, a, (const _Float16 *)fstar, ij, masks, n,
+                           per_point, partials);
+        break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_partials(const double *partials, int64_t nblocks, int32_t ncomp, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(reduce_partials, dim3(1), dim3(kBlockRows), 0, s, partials, nblocks, ncomp, out);
+    return hipGetLastError();
+}
+
+}  // namespace lbm

@@ -1,0 +1,185 @@
+"""GPU parity of the FP64 drop-in path against the CPU oracle (the
+reference's Serial policy semantics), through the C ABI.
+
+Bar: max |P_gpu - P_oracle| <= 1e-12 over wet-node population slots after
+1000 steps (the engine is built to match bit for bit; the tests report
+whether it does), plus rho/u of the macro fields, lift/drag and the FP64
+reductions.  Wall/obstacle slots are excluded: the reference keeps
+order-dependent scratch there (never read by a wet node)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import lbm_amd as L
+import oracle as O
+import workloads as W
+from conftest import ROOT
+from helpers import max_abs_wet, oracle_from, wet_mask
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+THREADS = min(16, os.cpu_count() or 1)
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def compare_state(gpu, orc, cls, what):
+    pg, po = gpu.populations(), orc.populations()
+    err = max_abs_wet(pg, po, cls)
+    exact = err == 0.0
+    say(f"  {what}: max|dP| = {err:.3e} ({'bit-exact' if exact else 'not bit-exact'})")
+    assert err <= TOL, f"{what}: {err}"
+    return err
+
+
+def run_pair(wl, steps, lift_every=0, values=None):
+    """Steps the GPU engine and the oracle side by side; values(i) may
+    return per-step inlet/outlet values (dict per list) like the reference's
+    update functions."""
+    say(f"case {wl.name}: {wl.nr}x{wl.nc}, {steps} steps")
+    O.set_threads(THREADS)
+    orc = oracle_from(wl)
+    gpu = wl.make_lattice(L.F64)
+    lifts = []
+    for i in range(steps):
+        if values is not None:
+            v = values(i)
+            for which, kw in v.items():
+                orc.set_open_values(which, **kw)
+                gpu.set_open_values(which, **kw)
+        flags = L.STEP_MACRO if i == steps - 1 else 0
+        if values is None and lift_every == 0:
+            gpu.step(steps, L.STEP_MACRO)
+            orc.step(steps)
+            break
+        gpu.step(1, flags)
+        orc.step(1)
+        if lift_every and i > 0 and i % lift_every == 0:
+            lg = gpu.lift_drag(per_point=True)
+            lo = orc.lift_drag(per_point=True)
+            assert np.max(np.abs(lg[2] - lo[2])) <= TOL
+            assert abs(lg[0] - lo[0]) <= 1e-12 * max(1.0, abs(lo[0])) and abs(lg[1] - lo[1]) <= 1e-12 * max(1.0, abs(lo[1]))
+            lifts.append((lo[0], lo[1]))
+    assert gpu.time == steps == orc.time
+    return gpu, orc, lifts
+
+
+def check_macro(gpu, orc, cls):
+    rg, ug = gpu.macro()
+    ro, uo = orc.macro()
+    m = (cls == L.FLUID) | (cls == L.INLET) | (cls == L.OUTLET)
+    assert np.max(np.abs(rg[m] - ro[m])) <= TOL
+    assert np.max(np.abs(ug[:, m] - uo[:, m])) <= TOL
+
+
+def test_cavity129_bgk_1000_steps():
+    wl = W.cavity(129)
+    gpu, orc, _ = run_pair(wl, 1000)
+    compare_state(gpu, orc, wl.node_class, "cavity129 BGK")
+    check_macro(gpu, orc, wl.node_class)
+    red_g, red_o = gpu.reduce(), orc.reduce()
+    assert np.allclose(red_g, red_o, rtol=1e-13, atol=1e-15)
+
+
+def test_channel_trt_cylinder_lift_drag():
+    wl = W.channel(64, 256, radius=6, centre=(33, 51))
+    gpu, orc, lifts = run_pair(wl, 1000, lift_every=100)
+    assert len(lifts) == 9
+    compare_state(gpu, orc, wl.node_class, "channel TRT + pressure outlet")
+    check_macro(gpu, orc, wl.node_class)
+    ag, ao = gpu.after_collision(), orc.after_collision()
+    assert max_abs_wet(ag, ao, wl.node_class) <= TOL
+
+
+def test_wing_time_varying_inlet_noop_outlet():
+    """examples/7_FlatBottomFoil.cpp: 100 x 400 channel, wing1.txt obstacle
+    file, TRT (0.6, 1.67), inlet u = 0.15 (1 - exp(-0.001 t)), no-op outlet,
+    lift/drag every 100 steps."""
+    nr, nc = 100, 400
+    cls = np.full((nc, nr), L.FLUID, dtype=np.uint8)
+    cls[:, 0] = cls[:, -1] = L.BOUNDARY
+    cls[0, :] = cls[-1, :] = L.BOUNDARY
+    cls[0, 1:nr - 1] = L.INLET
+    cls[-1, 1:nr - 1] = L.OUTLET
+    pts = np.loadtxt(os.path.join(GOLDEN, "wing1.txt"), skiprows=1, dtype=np.int64)
+    cls[pts[:, 1], pts[:, 0]] = L.OBSTACLE
+    wl = W.Workload(name="wing1", node_class=cls, collision=L.TRT, tau_even=0.6, tau_odd=1.67,
+                    inlet_policy=L.ZOUHE_VELOCITY, outlet_policy=L.OPEN_NOOP)
+    ni = int(np.count_nonzero(cls == L.INLET))
+
+    def values(t):
+        return {0: dict(ux=np.full(ni, 0.15 * (1 - np.exp(-t * 0.001))), uy=np.zeros(ni)),
+                1: dict(ux=0.0, uy=0.0)}
+
+    gpu, orc, lifts = run_pair(wl, 1000, lift_every=100, values=values)
+    compare_state(gpu, orc, cls, "wing1 time-varying inlet")
+    check_macro(gpu, orc, cls)
+
+
+def test_porous_pressure_bgk():
+    wl = W.porous(128, seed=11, rmin=3, rmax=8, nc=160)
+    gpu, orc, _ = run_pair(wl, 1000)
+    compare_state(gpu, orc, wl.node_class, "porous 128x160 pressure BGK")
+    check_macro(gpu, orc, wl.node_class)
+    red_g, red_o = gpu.reduce(), orc.reduce()
+    assert np.allclose(red_g, red_o, rtol=1e-12, atol=1e-14)
+
+
+def test_channel_full_size_101_steps():
+    wl = W.channel()  # 512 x 2048, cylinder r = 26 at (260, 410)
+    gpu, orc, _ = run_pair(wl, 101, lift_every=100)
+    compare_state(gpu, orc, wl.node_class, "channel 512x2048")
+
+
+def test_upload_download_roundtrip_and_state_machine():
+    wl = W.channel(40, 60, radius=5, centre=(20, 25))
+    gpu = L.Lattice(wl.node_class, **wl.lattice_kwargs())
+    with pytest.raises(L.LbmError):
+        gpu.step(1)                     # no populations yet
+    rng = np.random.default_rng(5)
+    p = rng.uniform(0.01, 0.2, size=(9, wl.nc, wl.nr))
+    gpu.upload_populations(p)
+    q = gpu.populations()
+    m = wet_mask(wl.node_class)
+    assert np.array_equal(q[:, m], p[:, m])
+    assert np.all(q[:, ~m] == 0.0)
+    with pytest.raises(L.LbmError):
+        gpu.after_collision()           # no step yet
+    # an uploaded state steps like the oracle from the same state
+    orc = O.OracleLattice(wl.node_class, **wl.lattice_kwargs())
+    orc.set_populations(p)
+    gpu.step(20)
+    orc.step(20)
+    assert max_abs_wet(gpu.populations(), orc.populations(), wl.node_class) <= TOL
+
+
+def test_nan_is_reported():
+    wl = W.cavity(33)
+    gpu = wl.make_lattice()
+    p = gpu.populations()
+    p[4, 10, 12] = np.nan
+    gpu.upload_populations(p)
+    gpu.step(3)
+    with pytest.raises(L.LbmError) as e:
+        gpu.step(1, L.STEP_CHECK)
+    assert e.value.code == 3
+
+
+def test_invalid_geometries_are_rejected():
+    cls = L.geometry_channel(16, 16)
+    bad = cls.copy()
+    bad[5, 5] = L.INLET                 # inlet inside the domain
+    with pytest.raises(L.LbmError) as e:
+        L.Lattice(bad)
+    assert e.value.code == 1
+    bad = cls.copy()
+    bad[5, 5] = L.SOLID                 # SOLID next to fluid
+    with pytest.raises(L.LbmError) as e:
+        L.Lattice(bad)
+    assert e.value.code == 5
