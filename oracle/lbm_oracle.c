@@ -151,6 +151,7 @@ orc_lattice *orc_create(int64_t nr, int64_t nc, const uint8_t *cls, int collisio
     L->rho = (double *)calloc(n, sizeof(double));
     L->u = (double *)calloc(2 * n, sizeof(double));
     if (!L->P || !L->E || !L->A || !L->rho || !L->u) { orc_destroy(L); *err = 2; return NULL; }
+    for (size_t k = 0; k < n; ++k) L->rho[k] = 1.0; /* Lattice.hpp:295, also before the first orc_init */
 
     int64_t cnt[6] = {0};
     for (size_t k = 0; k < n; ++k) {
