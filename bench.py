@@ -72,6 +72,9 @@ class HipEvents:
 def build_workload(name):
     if name == "porous8192":
         return W.porous(8192)
+    if name.startswith("open"):
+        # the porous set-up without obstacles (pressure-driven channel)
+        return W.porous(int(name[4:]), solid=0.0)
     if name.startswith("porous"):
         return W.porous(int(name[6:]))
     if name == "channel":

@@ -112,6 +112,7 @@ StepArgs make_args(const lbm_ctx *c)
     a.outlet_policy = c->desc.outlet_policy;
     a.collision = c->desc.collision;
     a.segs = (int32_t)c->segs;
+    a.esize = (int32_t)c->esize;
     // The reference recomputes t = 1.0 / tau inside every call
     // (BGKCollision.hpp:140-141) and (1.0/tau_even) etc. inside every
     // expression (TRTCollision.hpp:155-188); IEEE division makes these the
@@ -121,6 +122,12 @@ StepArgs make_args(const lbm_ctx *c)
     a.trt_inv_te = 1.0 / c->desc.tau_even;
     a.trt_a = 0.5 * (1.0 / c->desc.tau_even);
     a.trt_b = 0.5 * (1.0 / c->desc.tau_odd);
+    if (c->desc.collision == LBM_BGK) {
+        a.sh_a = a.sh_b = (float)(0.5 / c->desc.tau);
+    } else {
+        a.sh_a = (float)a.trt_a;
+        a.sh_b = (float)a.trt_b;
+    }
     a.scale = c->scale;
     a.inv_scale = 1.f / c->scale;
     a.flag = c->d_flag;
@@ -332,6 +339,13 @@ int lbm_set_geometry(lbm_ctx *c, const uint8_t *cls)
     c->wl = nrw + 2;
     c->E = 2 * nc + 2 * nr;
     c->segs = (c->ldr + kBlockRows - 1) / kBlockRows;
+    // The kernels address a population as a plane base pointer plus a
+    // 32-bit byte offset spanning up to three planes.
+    if ((uint64_t)3 * (uint64_t)c->plane * c->esize >= (1ull << 32)) {
+        c->nr = c->nc = 0;
+        return fail(LBM_ERR_UNSUPPORTED, "lattice of %lld x %lld nodes exceeds the 32-bit plane addressing "
+                    "(3 * plane * element size must stay below 4 GiB)", (long long)nr, (long long)nc);
+    }
     // host tables
     std::vector<uint64_t> wet((size_t)((nc + 2) * c->wl), 0);
     std::vector<uint8_t> edge((size_t)c->E, kEdgePlain);

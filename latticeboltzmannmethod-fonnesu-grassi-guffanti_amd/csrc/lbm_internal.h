@@ -41,19 +41,30 @@ struct StepArgs {
     Geometry g;
     const void *fin;      // state read by the step (F* of the previous step, or P)
     void *fout;           // F* written by the step
+    // Plane base pointers of fin / fout (plane d at fin + d * plane
+    // elements), passed explicitly so the kernels address every plane as
+    // a scalar base plus a 32-bit vector offset.
+    const void *fin_plane[9];
+    void *fout_plane[9];
     const void *frozen;   // [9][E] populations with out-of-domain source, storage type
     const double *bc_prev;  // [3][E]: ux, uy, rho prescribed at the previous step
     const double *bc_cur;   // [3][E]: values prescribed at this step
     int32_t inlet_policy, outlet_policy;
     int32_t collision;      // 0 BGK, 1 TRT
     int32_t segs;           // row segments (of kBlockRows) per column
+    int32_t esize;          // bytes per stored population
+    int32_t wpc;            // step kernel: waves per column
     int64_t nblocks;
     double bgk_t, bgk_omt;              // 1/tau, 1 - 1/tau
     double trt_inv_te, trt_a, trt_b;    // 1/tau+, 0.5/tau+, 0.5/tau-
+    float sh_a, sh_b;                   // shifted paths: 0.5/tau+, 0.5/tau- (BGK: both 0.5/tau)
     float scale, inv_scale;             // FP16 storage scale
     double *macro;          // optional [3][NC][NR]: rho, ux, uy
     int32_t *flag;          // bit 0: NaN density seen
 };
+
+// Fills fin_plane / fout_plane from fin / fout.
+void set_planes(StepArgs &a);
 
 // Kernel launchers (lbm_kernels.hip).  All are asynchronous on `s`.
 hipError_t launch_step(int32_t storage, bool gather, const StepArgs &a, hipStream_t s);

@@ -35,12 +35,9 @@ namespace {
 __device__ constexpr int kOffI[9] = {0, 0, -1, 0, 1, -1, -1, 1, 1};
 __device__ constexpr int kOffJ[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
 __device__ constexpr int kOpp[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
-// Lattice weights (DefaultEquilibrium.hpp:68-76) and the physical
-// velocity of each direction: x along +column, y along -row.
+// Lattice weights (DefaultEquilibrium.hpp:68-76).
 __device__ constexpr double kW[9] = {4.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0,
                                       1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
-__device__ constexpr int kCx[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
-__device__ constexpr int kCy[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
 
 // Boundary types of the reference's btypes enum (aliases.hpp:28-38).
 enum : int { BT_NONE = 0, BT_TOP, BT_RIGHT, BT_BOTTOM, BT_LEFT, BT_CORNER };
@@ -96,52 +93,133 @@ __device__ __forceinline__ Tile tile_of(const StepArgs &a)
     return t;
 }
 
-// Wet bits of the 3x3 neighbourhood.  The three words of each of the three
-// columns are wave-uniform; bit (row) selects the lane.  Returns the mask
-// of directions whose source is wet in bits 1..8 and the node's own bit in
-// bit 0.
-__device__ __forceinline__ uint32_t neighbour_mask(const Geometry &g, const Tile &t)
+// Wet bits of the 3x3 neighbourhood of a wave: for each of the columns
+// j-1, j, j+1 the words whose bit r is the wet bit of row i0+r-1 (up),
+// i0+r (mid) and i0+r+1 (dn).  All wave-uniform (scalar loads).
+struct Nbhd {
+    uint64_t up[3], mid[3], dn[3];
+};
+
+__device__ __forceinline__ Nbhd load_nbhd(const Geometry &g, const Tile &t)
 {
+    Nbhd h;
     int64_t k = t.i0 >> 6;
     const uint64_t *base = g.wet + (int64_t)t.j * g.wl + k;  // column j-1, word k-1
-    uint64_t up[3], mid[3], dn[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const uint64_t *col = base + c * g.wl;
         uint64_t w0 = col[0], w1 = col[1], w2 = col[2];
-        up[c] = (w1 << 1) | (w0 >> 63);   // bit r = row r-1
-        mid[c] = w1;                      // bit r = row r
-        dn[c] = (w1 >> 1) | (w2 << 63);   // bit r = row r+1
+        h.up[c] = (w1 << 1) | (w0 >> 63);
+        h.mid[c] = w1;
+        h.dn[c] = (w1 >> 1) | (w2 << 63);
     }
-    int l = t.lane;
-    uint32_t m = (uint32_t)((mid[1] >> l) & 1);
-    m |= (uint32_t)((mid[0] >> l) & 1) << 1;  // d1: source (i, j-1)
-    m |= (uint32_t)((dn[1] >> l) & 1) << 2;   // d2: source (i+1, j)
-    m |= (uint32_t)((mid[2] >> l) & 1) << 3;  // d3: source (i, j+1)
-    m |= (uint32_t)((up[1] >> l) & 1) << 4;   // d4: source (i-1, j)
-    m |= (uint32_t)((dn[0] >> l) & 1) << 5;   // d5: source (i+1, j-1)
-    m |= (uint32_t)((dn[2] >> l) & 1) << 6;   // d6: source (i+1, j+1)
-    m |= (uint32_t)((up[2] >> l) & 1) << 7;   // d7: source (i-1, j+1)
-    m |= (uint32_t)((up[0] >> l) & 1) << 8;   // d8: source (i-1, j-1)
+    return h;
+}
+
+// True when every node of the wave and every source of every node is wet:
+// no bounce-back, no perimeter node (guard bits are zero), plain streaming.
+__device__ __forceinline__ bool wave_is_bulk(const Nbhd &h)
+{
+    uint64_t all = ~0ull;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) all &= h.up[c] & h.mid[c] & h.dn[c];
+    return all == ~0ull;
+}
+
+// Per-lane mask: bit 0 = the node itself is wet, bit d = the source of
+// direction d is wet.
+__device__ __forceinline__ uint32_t lane_mask(const Nbhd &h, int l)
+{
+    uint32_t m = (uint32_t)((h.mid[1] >> l) & 1);
+    m |= (uint32_t)((h.mid[0] >> l) & 1) << 1;  // d1: source (i, j-1)
+    m |= (uint32_t)((h.dn[1] >> l) & 1) << 2;   // d2: source (i+1, j)
+    m |= (uint32_t)((h.mid[2] >> l) & 1) << 3;  // d3: source (i, j+1)
+    m |= (uint32_t)((h.up[1] >> l) & 1) << 4;   // d4: source (i-1, j)
+    m |= (uint32_t)((h.dn[0] >> l) & 1) << 5;   // d5: source (i+1, j-1)
+    m |= (uint32_t)((h.dn[2] >> l) & 1) << 6;   // d6: source (i+1, j+1)
+    m |= (uint32_t)((h.up[2] >> l) & 1) << 7;   // d7: source (i-1, j+1)
+    m |= (uint32_t)((h.up[0] >> l) & 1) << 8;   // d8: source (i-1, j-1)
     return m;
+}
+
+__device__ __forceinline__ uint32_t neighbour_mask(const Geometry &g, const Tile &t)
+{
+    return lane_mask(load_nbhd(g, t), t.lane);
+}
+
+// Element access through a wave-uniform base pointer and a 32-bit byte
+// offset, the addressing form global loads/stores take natively (scalar
+// base + vector offset).  The context only accepts lattices whose
+// population offsets (up to three planes) stay below 2^32 bytes.
+template <typename T>
+__device__ __forceinline__ T ld(const T *__restrict__ base, uint32_t byte_off)
+{
+    return *(const T *)((const char *)base + byte_off);
+}
+
+template <typename T>
+__device__ __forceinline__ void st(T *__restrict__ base, uint32_t byte_off, T v)
+{
+    *(T *)((char *)base + byte_off) = v;
+}
+
+// Byte offset of the source of direction d relative to the node (streaming).
+template <typename T>
+__device__ __forceinline__ int32_t src_off(int d, uint32_t L)
+{
+    const int32_t E = (int32_t)sizeof(T), l = (int32_t)L;
+    switch (d) {
+    case 1: return -l;
+    case 2: return E;
+    case 3: return l;
+    case 4: return -E;
+    case 5: return E - l;
+    case 6: return E + l;
+    case 7: return -E + l;
+    case 8: return -E - l;
+    default: return 0;
+    }
+}
+
+// Streams the 9 populations of a node whose sources are all wet.
+template <typename T>
+__device__ __forceinline__ void gather_bulk(const StepArgs &a, int64_t n, T *p)
+{
+    const uint32_t b = (uint32_t)n * (uint32_t)sizeof(T), L = (uint32_t)a.g.ldr * (uint32_t)sizeof(T);
+#pragma unroll
+    for (int d = 0; d < 9; ++d) p[d] = ld((const T *)a.fin_plane[d], b + (uint32_t)src_off<T>(d, L));
 }
 
 // Streams the 9 populations of an interior wet node: one load per
 // direction, its address chosen between the neighbour (stream) and the
-// node's own opposite slot (bounce-back).
+// node's own opposite slot (bounce-back).  Opposite directions are two
+// planes apart, so both candidates are reached from the lower plane of the
+// pair with a non-negative 32-bit offset.
 template <typename T>
-__device__ __forceinline__ void gather_interior(const T *__restrict__ F, const Geometry &g, int64_t n, uint32_t m,
-                                                T *p)
+__device__ __forceinline__ void gather_interior(const StepArgs &a, int64_t n, uint32_t m, T *p)
 {
-    const int64_t P = g.plane, L = g.ldr;
-    const int64_t src[9] = {0, n - L, n + 1, n + L, n - 1, n + 1 - L, n + 1 + L, n - 1 + L, n - 1 - L};
-    p[0] = F[n];
+    const uint32_t b = (uint32_t)n * (uint32_t)sizeof(T), L = (uint32_t)a.g.ldr * (uint32_t)sizeof(T);
+    const uint32_t twoP = 2u * (uint32_t)a.g.plane * (uint32_t)sizeof(T);
+    p[0] = ld((const T *)a.fin_plane[0], b);
 #pragma unroll
     for (int d = 1; d < 9; ++d) {
-        bool wet = (m >> d) & 1;
-        int64_t idx = wet ? d * P + src[d] : kOpp[d] * P + n;
-        p[d] = F[idx];
+        const bool wet = (m >> d) & 1;
+        const int o = kOpp[d];
+        const bool low = d < o;  // d is the lower plane of its pair
+        const T *base = (const T *)a.fin_plane[low ? d : o];
+        uint32_t nb = b + (uint32_t)src_off<T>(d, L) + (low ? 0u : twoP);
+        uint32_t ob = b + (low ? twoP : 0u);
+        p[d] = ld(base, wet ? nb : ob);
     }
+}
+
+// Writes the 9 post-collision populations of node n.
+template <typename T>
+__device__ __forceinline__ void store9(const StepArgs &a, int64_t n, const T *o)
+{
+    const uint32_t b = (uint32_t)n * (uint32_t)sizeof(T);
+#pragma unroll
+    for (int d = 0; d < 9; ++d) st((T *)a.fout_plane[d], b, o[d]);
 }
 
 // Same for a perimeter node: sources outside the domain read the frozen
@@ -278,7 +356,7 @@ __device__ __forceinline__ int load_p64(const StepArgs &a, const double *__restr
 {
     int policy = -1;
     if (!edge) {
-        if (GATHER) gather_interior(F, a.g, n, m, p);
+        if (GATHER) gather_interior(a, n, m, p);
         else
 #pragma unroll
             for (int d = 0; d < 9; ++d) p[d] = F[d * a.g.plane + n];
@@ -301,25 +379,15 @@ __device__ __forceinline__ int load_p64(const StepArgs &a, const double *__restr
     return policy;
 }
 
-template <bool GATHER>
-__global__ __launch_bounds__(kBlockRows) void step_f64(StepArgs a)
+// Moments, equilibrium, collision and store of one wet node in FP64, in
+// the reference's expression order (update_macro, calc_equilibrium,
+// collide: Lattice.hpp:351-382).  Fluid and pressure nodes take their
+// moments; velocity-prescribed and no-op nodes keep rho = 1
+// (Lattice.hpp:295; the Zou-He density write is lost on a by-value copy,
+// ZouHePolicy.hpp:58,71) and the prescribed u.
+__device__ __forceinline__ void collide_store_f64(const StepArgs &a, const double *p, int policy, int64_t e, int i,
+                                                  int j, int64_t n)
 {
-    Tile t = tile_of(a);
-    if (!t.valid) return;
-    uint32_t m = neighbour_mask(a.g, t);
-    if (!(m & 1)) return;
-    const int i = t.i, j = t.j;
-    const int64_t n = (int64_t)j * a.g.ldr + i;
-    const bool edge = (i == 0) | (i == a.g.nr - 1) | (j == 0) | (j == a.g.nc - 1);
-    const double *__restrict__ F = (const double *)a.fin;
-    double p[9];
-    int64_t e = 0;
-    int policy = load_p64<GATHER>(a, F, i, j, n, m, edge, e, p);
-
-    // update_nodes + update_macro (Lattice.hpp:351-357): fluid and
-    // pressure nodes take their moments, velocity-prescribed nodes keep
-    // rho = 1 (Lattice.hpp:295; the Zou-He density write is lost on a
-    // by-value copy, ZouHePolicy.hpp:58,71) and the prescribed u.
     double rho, ux, uy;
     if (policy == kVelocity || policy == kNoop) {
         rho = 1.0;
@@ -354,9 +422,22 @@ __global__ __launch_bounds__(kBlockRows) void step_f64(StepArgs a)
             o[d] = p[d] - ca * (p[d] + p[q] - eq[d] - eq[q]) - cb * (p[d] - p[q] - eq[d] + eq[q]);
         }
     }
-    double *__restrict__ G = (double *)a.fout;
-#pragma unroll
-    for (int d = 0; d < 9; ++d) G[d * a.g.plane + n] = o[d];
+    store9(a, n, o);
+}
+
+// One node of any kind (perimeter, open boundary, bounce-back).
+template <bool GATHER>
+__device__ __forceinline__ void node_general_f64(const StepArgs &a, const Tile &t, const Nbhd &h)
+{
+    uint32_t m = lane_mask(h, t.lane);
+    if (!(m & 1)) return;
+    const int i = t.i, j = t.j;
+    const int64_t n = (int64_t)j * a.g.ldr + i;
+    const bool edge = (i == 0) | (i == a.g.nr - 1) | (j == 0) | (j == a.g.nc - 1);
+    double p[9];
+    int64_t e = 0;
+    int policy = load_p64<GATHER>(a, (const double *)a.fin, i, j, n, m, edge, e, p);
+    collide_store_f64(a, p, policy, e, i, j, n);
 }
 
 // ------------------------------------------------------- shifted FP32 --
@@ -375,49 +456,38 @@ __device__ __forceinline__ float from_g<float>(float g, float) { return g; }
 template <>
 __device__ __forceinline__ _Float16 from_g<_Float16>(float g, float scale) { return (_Float16)(g * scale); }
 
-// Compensated running sum (Knuth two-sum; the error terms are added up
-// separately and folded in at the end).
-struct CSum {
-    float s = 0.f, c = 0.f;
-    __device__ __forceinline__ void add(float x)
-    {
-        float t = s + x;
-        float bp = t - s;
-        c += (s - (t - bp)) + (x - bp);
-        s = t;
-    }
-    __device__ __forceinline__ float value() const { return s + c; }
-};
-
 // Zou-He closures on shifted populations g = f - w.  The weights of the
 // known sets sum to 1 and opposite weights are equal, so the closures keep
 // their form with f -> g and rho -> 1 + (sum) as below.  Same scalar
 // structure as the FP64 closures.
 __device__ __forceinline__ void zouhe_velocity32(int bt, float *g, float ux, float uy)
 {
+    // rho - 1 is formed directly, e.g. TOP: rho = (1 + G) / (1 + uy) gives
+    // rho - 1 = (G - uy) / (1 + uy); rho * u = u + (rho - 1) * u.  Rounding
+    // 1 + G first would cost the low bits of G.
     float g0 = g[0], g1 = g[1], g2 = g[2], g3 = g[3], g4 = g[4], g5 = g[5], g6 = g[6], g7 = g[7], g8 = g[8];
-    float rho, ru, rv;
+    float dr, ru, rv;
     if (bt == BT_TOP) {
-        rho = (1.f + (g0 + g1 + g3 + 2.f * (g2 + g5 + g6))) / (1.f + uy);
-        ru = rho * ux; rv = rho * uy;
+        dr = ((g0 + g1 + g3 + 2.f * (g2 + g5 + g6)) - uy) / (1.f + uy);
+        ru = __builtin_fmaf(dr, ux, ux); rv = __builtin_fmaf(dr, uy, uy);
         g4 = g2 - (2.f / 3.f) * rv;
         g7 = g5 - (1.f / 6.f) * rv - 0.5f * ru + 0.5f * (g1 - g3);
         g8 = g6 - (1.f / 6.f) * rv + 0.5f * ru - 0.5f * (g1 - g3);
     } else if (bt == BT_BOTTOM) {
-        rho = (1.f + (g0 + g1 + g3 + 2.f * (g4 + g7 + g8))) / (1.f - uy);
-        ru = rho * ux; rv = rho * uy;
+        dr = ((g0 + g1 + g3 + 2.f * (g4 + g7 + g8)) + uy) / (1.f - uy);
+        ru = __builtin_fmaf(dr, ux, ux); rv = __builtin_fmaf(dr, uy, uy);
         g2 = g4 + (2.f / 3.f) * rv;
         g5 = g7 + (1.f / 6.f) * rv + 0.5f * ru - 0.5f * (g1 - g3);
         g6 = g8 + (1.f / 6.f) * rv - 0.5f * ru + 0.5f * (g1 - g3);
     } else if (bt == BT_LEFT) {
-        rho = (1.f + (g0 + g2 + g4 + 2.f * (g3 + g7 + g6))) / (1.f - ux);
-        ru = rho * ux; rv = rho * uy;
+        dr = ((g0 + g2 + g4 + 2.f * (g3 + g7 + g6)) + ux) / (1.f - ux);
+        ru = __builtin_fmaf(dr, ux, ux); rv = __builtin_fmaf(dr, uy, uy);
         g1 = g3 + (2.f / 3.f) * ru;
         g5 = g7 + (1.f / 6.f) * ru + 0.5f * rv - 0.5f * (g2 - g4);
         g8 = g6 + (1.f / 6.f) * ru - 0.5f * rv + 0.5f * (g2 - g4);
     } else if (bt == BT_RIGHT) {
-        rho = (1.f + (g0 + g2 + g4 + 2.f * (g1 + g5 + g8))) / (1.f + ux);
-        ru = rho * ux; rv = rho * uy;
+        dr = ((g0 + g2 + g4 + 2.f * (g1 + g5 + g8)) - ux) / (1.f + ux);
+        ru = __builtin_fmaf(dr, ux, ux); rv = __builtin_fmaf(dr, uy, uy);
         g3 = g1 - (2.f / 3.f) * ru;
         g7 = g5 - (1.f / 6.f) * ru - 0.5f * rv + 0.5f * (g2 - g4);
         g6 = g8 - (1.f / 6.f) * ru + 0.5f * rv - 0.5f * (g2 - g4);
@@ -463,7 +533,7 @@ __device__ __forceinline__ int load_g(const StepArgs &a, const T *__restrict__ F
     T raw[9];
     int policy = -1;
     if (!edge) {
-        if (GATHER) gather_interior(F, a.g, n, m, raw);
+        if (GATHER) gather_interior(a, n, m, raw);
         else
 #pragma unroll
             for (int d = 0; d < 9; ++d) raw[d] = F[d * a.g.plane + n];
@@ -494,6 +564,20 @@ __device__ __forceinline__ int load_g(const StepArgs &a, const T *__restrict__ F
     return policy;
 }
 
+// Compensated running sum (Knuth two-sum per addition, errors folded in
+// at the end): the result is the correctly rounded sum in practice.
+struct CSum {
+    float s = 0.f, c = 0.f;
+    __device__ __forceinline__ void add(float x)
+    {
+        float t = s + x;
+        float bp = t - s;
+        c += (s - (t - bp)) + (x - bp);
+        s = t;
+    }
+    __device__ __forceinline__ float value() const { return s + c; }
+};
+
 // Moments of shifted populations: drho = sum g, j = sum c g, compensated.
 __device__ __forceinline__ void moments32(const float *g, float &drho, float &jx, float &jy)
 {
@@ -507,34 +591,33 @@ __device__ __forceinline__ void moments32(const float *g, float &drho, float &jx
     jy = y.value();
 }
 
-template <typename T, bool GATHER>
-__global__ __launch_bounds__(kBlockRows) void step_shifted(StepArgs a)
-{
-    Tile t = tile_of(a);
-    if (!t.valid) return;
-    uint32_t m = neighbour_mask(a.g, t);
-    if (!(m & 1)) return;
-    const int i = t.i, j = t.j;
-    const int64_t n = (int64_t)j * a.g.ldr + i;
-    const bool edge = (i == 0) | (i == a.g.nr - 1) | (j == 0) | (j == a.g.nc - 1);
-    const T *__restrict__ F = (const T *)a.fin;
-    float g[9];
-    int64_t e = 0;
-    int policy = load_g<T, GATHER>(a, F, i, j, n, m, edge, e, g);
+// Physical velocity of each direction (x along +column, y along -row),
+// from the reference's moment sums (DefaultEquilibrium.hpp:148-149).
+__device__ constexpr float kCxf[9] = {0.f, 1.f, 0.f, -1.f, 0.f, 1.f, -1.f, -1.f, 1.f};
+__device__ constexpr float kCyf[9] = {0.f, 0.f, 1.f, 0.f, -1.f, 1.f, 1.f, -1.f, -1.f};
 
-    float drho, ux, uy, rho;
+// Moments, equilibrium, collision and store of one wet node on shifted
+// populations, FP32 arithmetic.  Every step is written to keep rounding
+// small relative to the stored values: compensated moment sums, an
+// equilibrium in which 1 + drho is never rounded, and a collision applied
+// to the (small) non-equilibrium part geq - g.
+template <typename T>
+__device__ __forceinline__ void collide_store_shifted(const StepArgs &a, const float *g, int policy, int64_t e, int i,
+                                                      int j, int64_t n)
+{
+    float drho, jx, jy;
+    moments32(g, drho, jx, jy);
+    float ux, uy;
     if (policy == kVelocity || policy == kNoop) {
+        // rho = 1 and the prescribed velocity (see collide_store_f64)
+        if (drho != drho) atomicOr(a.flag, 1);
         drho = 0.f;
-        rho = 1.f;
         ux = (float)a.bc_cur[e];
         uy = (float)a.bc_cur[a.g.E + e];
-        float s = g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6] + g[7] + g[8];
-        if (s != s) atomicOr(a.flag, 1);
     } else {
-        float jx, jy;
-        moments32(g, drho, jx, jy);
-        rho = 1.f + drho;
-        float rinv = 1.f / rho;
+        // Correctly rounded reciprocal: an approximate one is biased and
+        // acts as a small systematic drag on the momentum.
+        float rinv = 1.f / (1.f + drho);
         ux = jx * rinv;
         uy = jy * rinv;
         if (drho != drho) atomicOr(a.flag, 1);
@@ -545,35 +628,136 @@ __global__ __launch_bounds__(kBlockRows) void step_shifted(StepArgs a)
         a.macro[N + q] = ux;
         a.macro[2 * N + q] = uy;
     }
-    // Shifted equilibrium geq_d = feq_d - w_d
-    //   = w_d * (drho + rho * (3 cu + 4.5 cu^2 - 1.5 u^2)).
+    // geq_d = feq_d - w_d = w_d (drho + (1 + drho) poly_d),
+    // poly_d = 3 cu + 4.5 cu^2 - 1.5 u^2, as w_d ((poly + drho poly) + drho).
     const float u15 = 1.5f * (ux * ux + uy * uy);
-    float geq[9];
+    float n_[9];
 #pragma unroll
     for (int d = 0; d < 9; ++d) {
-        float cu = (float)kCx[d] * ux + (float)kCy[d] * uy;
+        float cu = kCxf[d] * ux + kCyf[d] * uy;
         float poly = __builtin_fmaf(cu, __builtin_fmaf(4.5f, cu, 3.f), -u15);
-        geq[d] = (float)kW[d] * __builtin_fmaf(rho, poly, drho);
+        float geq = (float)kW[d] * (__builtin_fmaf(drho, poly, poly) + drho);
+        n_[d] = geq - g[d];
     }
+    // TRT on opposite pairs of n = geq - g:
+    //   o_d = g_d + ca (n_d + n_o) + cb (n_d - n_o)
+    // (BGK: ca = cb = omega / 2, i.e. o_d = g_d + omega n_d).
+    const float ca = a.sh_a, cb = a.sh_b;
     float o[9];
-    if (a.collision == 0) {
-        const float om = (float)a.bgk_t;
+    o[0] = __builtin_fmaf(2.f * ca, n_[0], g[0]);
 #pragma unroll
-        for (int d = 0; d < 9; ++d) o[d] = __builtin_fmaf(om, geq[d] - g[d], g[d]);
-    } else {
-        const float ca = (float)a.trt_a, cb = (float)a.trt_b;
-        o[0] = __builtin_fmaf((float)a.trt_inv_te, geq[0] - g[0], g[0]);
+    for (int d = 1; d < 9; ++d) {
+        const int q = kOpp[d];
+        o[d] = g[d] + __builtin_fmaf(ca, n_[d] + n_[q], cb * (n_[d] - n_[q]));
+    }
+    T raw[9];
 #pragma unroll
-        for (int d = 1; d < 9; ++d) {
-            const int q = kOpp[d];
-            float sym = (g[d] + g[q]) - (geq[d] + geq[q]);
-            float asym = (g[d] - g[q]) - (geq[d] - geq[q]);
-            o[d] = g[d] - __builtin_fmaf(ca, sym, cb * asym);
+    for (int d = 0; d < 9; ++d) raw[d] = from_g<T>(o[d], a.scale);
+    store9(a, n, raw);
+}
+
+template <typename T, bool GATHER>
+__device__ __forceinline__ void node_general_shifted(const StepArgs &a, const Tile &t, const Nbhd &h)
+{
+    uint32_t m = lane_mask(h, t.lane);
+    if (!(m & 1)) return;
+    const int i = t.i, j = t.j;
+    const int64_t n = (int64_t)j * a.g.ldr + i;
+    const bool edge = (i == 0) | (i == a.g.nr - 1) | (j == 0) | (j == a.g.nc - 1);
+    float g[9];
+    int64_t e = 0;
+    int policy = load_g<T, GATHER>(a, (const T *)a.fin, i, j, n, m, edge, e, g);
+    collide_store_shifted<T>(a, g, policy, e, i, j, n);
+}
+
+// ------------------------------------------------------------ step kernel --
+
+template <typename T> struct ComputeOf { using type = float; };
+template <> struct ComputeOf<double> { using type = double; };
+
+// The time-step kernel.  Each wave owns R consecutive 64-row segments of
+// one column.  Waves touching the domain perimeter (and the collide-only
+// first step) run every node through the general path; all other waves
+// first issue the loads of all R segments -- plain streaming for a segment
+// whose 3x3 neighbourhood is all wet, masked bounce-back otherwise -- and
+// only then collide and store, so R*9 loads per lane are in flight.
+template <typename T, bool GATHER, int R>
+__global__ __launch_bounds__(kBlockRows) void step_kernel(StepArgs a)
+{
+    using V = typename ComputeOf<T>::type;
+    const int64_t b = xcd_remap(blockIdx.x, a.nblocks);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int64_t gw = b * (kBlockRows / 64) + wave;
+    const int j = (int)(gw / a.wpc);
+    if (j >= a.g.nc) return;
+    const int64_t row0 = (gw - (int64_t)j * a.wpc) * (64 * R);
+    if (row0 >= a.g.ldr) return;
+    const int nr = a.g.nr;
+    const bool perim = (j == 0) | (j == a.g.nc - 1) | (row0 == 0) | (row0 <= nr - 1 && nr - 1 < row0 + 64 * R);
+    if (perim || !GATHER) {
+#pragma unroll 1
+        for (int r = 0; r < R; ++r) {
+            Tile t;
+            t.j = j;
+            t.i0 = row0 + 64 * r;
+            t.lane = lane;
+            t.i = (int)t.i0 + lane;
+            if (t.i0 >= a.g.ldr) break;
+            const Nbhd h = load_nbhd(a.g, t);
+            if constexpr (sizeof(T) == 8) node_general_f64<GATHER>(a, t, h);
+            else node_general_shifted<T, GATHER>(a, t, h);
+        }
+        return;
+    }
+    V p[R][9];
+    bool act[R], any[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        Tile t;
+        t.j = j;
+        t.i0 = row0 + 64 * r;
+        t.lane = lane;
+        t.i = (int)t.i0 + lane;
+        act[r] = false;
+        any[r] = false;
+        if (t.i0 < a.g.ldr) {
+            const Nbhd h = load_nbhd(a.g, t);
+            const int64_t n = (int64_t)j * a.g.ldr + t.i;
+            any[r] = h.mid[1] != 0;
+            T raw[9];
+            if (wave_is_bulk(h)) {
+                gather_bulk(a, n, raw);
+                act[r] = true;
+            } else {
+                uint32_t m = lane_mask(h, lane);
+                act[r] = m & 1;
+                if (act[r]) gather_interior(a, n, m, raw);
+            }
+#pragma unroll
+            for (int d = 0; d < 9; ++d) {
+                if constexpr (sizeof(T) == 8) p[r][d] = raw[d];
+                else p[r][d] = to_g<T>(raw[d], a.inv_scale);
+            }
         }
     }
-    T *__restrict__ G = (T *)a.fout;
 #pragma unroll
-    for (int d = 0; d < 9; ++d) G[d * a.g.plane + n] = from_g<T>(o[d], a.scale);
+    for (int r = 0; r < R; ++r) {
+        const int i = (int)row0 + 64 * r + lane;
+        const int64_t n = (int64_t)j * a.g.ldr + i;
+        if (act[r]) {
+            if constexpr (sizeof(T) == 8) collide_store_f64(a, p[r], -1, 0, i, j, n);
+            else collide_store_shifted<T>(a, p[r], -1, 0, i, j, n);
+        } else if (any[r] && i < a.g.nr) {
+            // Dry node in a wave with wet nodes: write a placeholder so the
+            // wave's stores cover whole cache lines (a partly written line
+            // costs HBM a read-modify-write).  Dry slots are never read.
+            T zero[9];
+#pragma unroll
+            for (int d = 0; d < 9; ++d) zero[d] = T(0);
+            store9(a, n, zero);
+        }
+    }
 }
 
 // ------------------------------------------------- readout / reductions --
@@ -783,26 +967,53 @@ __global__ __launch_bounds__(kBlockRows) void reduce_partials(const double *part
 
 int64_t blocks_of(const StepArgs &a) { return (int64_t)a.segs * a.g.nc; }
 
+
 template <typename K>
 hipError_t launch(K kernel, int64_t nb, hipStream_t s, StepArgs a)
 {
     a.nblocks = nb;
+    set_planes(a);
     hipLaunchKernelGGL(kernel, dim3((unsigned)nb), dim3(kBlockRows), 0, s, a);
     return hipGetLastError();
 }
 
 }  // namespace
 
+void set_planes(StepArgs &a)
+{
+    const size_t stride = (size_t)a.g.plane * (size_t)a.esize;
+    for (int d = 0; d < 9; ++d) {
+        a.fin_plane[d] = a.fin ? (const char *)a.fin + (size_t)d * stride : nullptr;
+        a.fout_plane[d] = a.fout ? (char *)a.fout + (size_t)d * stride : nullptr;
+    }
+}
+
+int32_t step_segments(int32_t storage)
+{
+    return storage == kF64 ? 2 : 4;
+}
+
+int64_t step_blocks(const StepArgs &a, int32_t R)
+{
+    const int64_t wpc = (a.g.ldr / 64 + R - 1) / R;
+    const int64_t waves = wpc * a.g.nc;
+    return (waves + (kBlockRows / 64) - 1) / (kBlockRows / 64);
+}
+
+template <typename T, int R>
+hipError_t launch_step_t(bool gather, StepArgs a, hipStream_t s)
+{
+    a.wpc = (int32_t)((a.g.ldr / 64 + R - 1) / R);
+    const int64_t nb = step_blocks(a, R);
+    return gather ? launch(step_kernel<T, true, R>, nb, s, a) : launch(step_kernel<T, false, R>, nb, s, a);
+}
+
 hipError_t launch_step(int32_t storage, bool gather, const StepArgs &a, hipStream_t s)
 {
-    int64_t nb = blocks_of(a);
     switch (storage) {
-    case kF64: return gather ? launch(step_f64<true>, nb, s, a) : launch(step_f64<false>, nb, s, a);
-    case kF32:
-        return gather ? launch(step_shifted<float, true>, nb, s, a) : launch(step_shifted<float, false>, nb, s, a);
-    default:
-        return gather ? launch(step_shifted<_Float16, true>, nb, s, a)
-                      : launch(step_shifted<_Float16, false>, nb, s, a);
+    case kF64: return launch_step_t<double, 2>(gather, a, s);
+    case kF32: return launch_step_t<float, 4>(gather, a, s);
+    default: return launch_step_t<_Float16, 4>(gather, a, s);
     }
 }
 
@@ -813,6 +1024,7 @@ hipError_t launch_readout(int32_t storage, bool gather, const StepArgs &a0, doub
 {
     StepArgs a = a0;
     a.nblocks = blocks_of(a);
+    set_planes(a);
     dim3 grid((unsigned)a.nblocks), block(kBlockRows);
     switch (storage) {
     case kF64:

@@ -81,7 +81,7 @@ def _l2rel(a, b, m):
     return float(np.sqrt(np.sum((a[..., m] - b[..., m]) ** 2) / np.sum(b[..., m] ** 2)))
 
 
-@pytest.mark.parametrize("storage,tol_u", [(L.F32_SHIFTED, 1e-5), (L.F16_SHIFTED, 5e-2)])
+@pytest.mark.parametrize("storage,tol_u", [(L.F32_SHIFTED, 3e-5), (L.F16_SHIFTED, 5e-2)])
 def test_shifted_storage_error_cavity(storage, tol_u):
     wl = W.cavity(129)
     say(f"case {wl.name}: storage {storage} vs FP64, 5000 steps")
