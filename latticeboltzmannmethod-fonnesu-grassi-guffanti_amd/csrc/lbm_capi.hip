@@ -58,6 +58,7 @@ struct lbm_ctx {
     bool has_geom = false;
     int64_t nr = 0, nc = 0, ldr = 0, plane = 0, wl = 0, E = 0, segs = 0;
     uint64_t *d_wet = nullptr;
+    int32_t *d_sb = nullptr;
     uint8_t *d_edge_kind = nullptr;
     void *d_frozen = nullptr;
     void *d_F[2] = {nullptr, nullptr};
@@ -102,6 +103,7 @@ StepArgs make_args(const lbm_ctx *c)
     a.g.ldr = c->ldr;
     a.g.plane = c->plane;
     a.g.wet = c->d_wet;
+    a.g.sb = c->d_sb;
     a.g.wl = c->wl;
     a.g.E = c->E;
     a.g.edge_kind = c->d_edge_kind;
@@ -137,6 +139,7 @@ StepArgs make_args(const lbm_ctx *c)
 void free_geometry(lbm_ctx *c)
 {
     (void)hipFree(c->d_wet); c->d_wet = nullptr;
+    (void)hipFree(c->d_sb); c->d_sb = nullptr;
     (void)hipFree(c->d_edge_kind); c->d_edge_kind = nullptr;
     (void)hipFree(c->d_frozen); c->d_frozen = nullptr;
     for (int k = 0; k < 2; ++k) {
@@ -335,17 +338,9 @@ int lbm_set_geometry(lbm_ctx *c, const uint8_t *cls)
     c->nc = nc;
     const int64_t nrw = (nr + 63) / 64;
     c->ldr = nrw * 64;
-    c->plane = ((c->ldr * nc + 63) / 64) * 64;
     c->wl = nrw + 2;
     c->E = 2 * nc + 2 * nr;
     c->segs = (c->ldr + kBlockRows - 1) / kBlockRows;
-    // The kernels address a population as a plane base pointer plus a
-    // 32-bit byte offset spanning up to three planes.
-    if ((uint64_t)3 * (uint64_t)c->plane * c->esize >= (1ull << 32)) {
-        c->nr = c->nc = 0;
-        return fail(LBM_ERR_UNSUPPORTED, "lattice of %lld x %lld nodes exceeds the 32-bit plane addressing "
-                    "(3 * plane * element size must stay below 4 GiB)", (long long)nr, (long long)nc);
-    }
     // host tables
     std::vector<uint64_t> wet((size_t)((nc + 2) * c->wl), 0);
     std::vector<uint8_t> edge((size_t)c->E, kEdgePlain);
@@ -364,6 +359,26 @@ int lbm_set_geometry(lbm_ctx *c, const uint8_t *cls)
             }
             if (k == LBM_INLET || k == LBM_OUTLET) edge[(size_t)edge_slot_host(i, j, nr, nc)] = k == LBM_INLET ? kEdgeInlet : kEdgeOutlet;
         }
+    // Compact storage: wet nodes only, column-major.  sb holds the compact
+    // index of the first wet node of every (column, 64-row word).
+    std::vector<int32_t> sb((size_t)((nc + 2) * c->wl), 0);
+    {
+        int64_t acc = 0;
+        for (int64_t j = 0; j < nc; ++j)
+            for (int64_t k = 0; k < nrw; ++k) {
+                size_t w = (size_t)((j + 1) * c->wl + 1 + k);
+                sb[w] = (int32_t)acc;
+                acc += __builtin_popcountll(wet[w]);
+            }
+    }
+    c->plane = std::max<int64_t>(64, ((c->n_wet + 63) / 64) * 64);
+    // The kernels address a population as a plane base pointer plus a
+    // 32-bit byte offset spanning up to three planes.
+    if ((uint64_t)3 * (uint64_t)c->plane * c->esize >= (1ull << 32)) {
+        c->nr = c->nc = 0;
+        return fail(LBM_ERR_UNSUPPORTED, "lattice with %lld wet nodes exceeds the 32-bit plane addressing "
+                    "(3 * wet nodes * element size must stay below 4 GiB)", (long long)c->n_wet);
+    }
     // list order of the reference: row-major scan
     for (int64_t i = 0; i < nr; ++i)
         for (int64_t j = 0; j < nc; ++j) {
@@ -393,6 +408,8 @@ int lbm_set_geometry(lbm_ctx *c, const uint8_t *cls)
     size_t fbytes = (size_t)(9 * c->plane) * c->esize;
     HIP_TRY(hipMalloc(&c->d_wet, wet.size() * sizeof(uint64_t)));
     HIP_TRY(hipMemcpy(c->d_wet, wet.data(), wet.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&c->d_sb, sb.size() * sizeof(int32_t)));
+    HIP_TRY(hipMemcpy(c->d_sb, sb.data(), sb.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMalloc(&c->d_edge_kind, (size_t)c->E));
     HIP_TRY(hipMemcpy(c->d_edge_kind, edge.data(), (size_t)c->E, hipMemcpyHostToDevice));
     HIP_TRY(hipMalloc(&c->d_frozen, (size_t)(9 * c->E) * c->esize));

@@ -17,12 +17,16 @@ enum EdgeKind : uint8_t { kEdgePlain = 0, kEdgeInlet = 1, kEdgeOutlet = 2 };
 constexpr int kBlockRows = 256;
 
 // Device-side description of the lattice.
-//  * Populations: 9 planes of `plane` elements; node (i, j) of plane d at
-//    d*plane + j*ldr + i, with ldr = NR rounded up to 64 (a wave covers one
-//    aligned 64-row run of a column).
+//  * Populations are stored for wet nodes only, in column-major order (rows
+//    ascending within a column), 9 planes of `plane` elements: the node with
+//    compact index c, direction d, sits at d*plane + c.
 //  * Wet mask: one bit per node, 64-bit words along rows, one guard word
 //    above and below every column and one guard column on each side, all
 //    zero: word (j, k) at wet[(j+1)*wl + (k+1)], bit i%64 of k = i/64.
+//  * sb (same layout as wet): compact index of the first wet node of
+//    (column j, word k); node (i, j) is at sb + popcount of the wet bits of
+//    its word below bit i%64.
+//  * ldr = NR rounded up to 64 (the row tiling of the kernels).
 //  * Edge slots: every perimeter node owns slot e (top row e = j, bottom
 //    row NC + j, left column 2NC + i, right column 2NC + NR + i); they hold
 //    the inlet/outlet kind, the prescribed values and the frozen
@@ -32,6 +36,7 @@ struct Geometry {
     int64_t ldr;
     int64_t plane;
     const uint64_t *wet;
+    const int32_t *sb;
     int64_t wl;
     int64_t E;
     const uint8_t *edge_kind;

@@ -16,15 +16,18 @@
 // push + bounce-back + Zou-He sequence of Lattice::perform_lbm
 // (Lattice.hpp:351-410) slot for slot.
 //
+// Storage is compact: only wet nodes are stored, in column-major order
+// (rows ascending within a column).  A node's index is the base of its
+// (column, 64-row segment) plus the number of wet rows above it in the
+// segment -- a popcount of the bit-packed wet mask below the lane
+// (v_mbcnt).  Neighbours one row up/down are the previous/next compact
+// entries, so every source index follows from three bases and three
+// popcounts per wave column.  A wave then touches only wet bytes.
+//
 // The FP64 path evaluates every expression in the reference's order with
 // no contraction (the library is built with -ffp-contract=off), so it
 // matches the Serial policy bit for bit.  The shifted paths store
 // g = f - w (FP32) or (f - w) * scale (FP16) and compute in FP32.
-//
-// Memory traffic per wet node and step: 9 loads + 9 stores of the storage
-// type plus the mask bits (read as wave-uniform words).  No LDS: every
-// source value is read by exactly one destination, so the only reuse is
-// the line shared by neighbouring waves of one column, which L2 holds.
 #include "lbm_internal.h"
 
 namespace lbm {
@@ -79,6 +82,8 @@ struct Tile {
     bool valid;
 };
 
+// Dense tiling used by the set-up/readout kernels: a work-group covers
+// kBlockRows consecutive rows of one column.
 __device__ __forceinline__ Tile tile_of(const StepArgs &a)
 {
     Tile t;
@@ -93,25 +98,30 @@ __device__ __forceinline__ Tile tile_of(const StepArgs &a)
     return t;
 }
 
-// Wet bits of the 3x3 neighbourhood of a wave: for each of the columns
-// j-1, j, j+1 the words whose bit r is the wet bit of row i0+r-1 (up),
-// i0+r (mid) and i0+r+1 (dn).  All wave-uniform (scalar loads).
+// The 3x3 wet neighbourhood of a wave and the compact-index bases of its
+// three columns, all wave-uniform (scalar loads): for the columns j-1, j,
+// j+1 the words whose bit r is the wet bit of row i0+r-1 (up), i0+r (mid)
+// and i0+r+1 (dn), and the compact index of the first wet node of the
+// segment (base).
 struct Nbhd {
     uint64_t up[3], mid[3], dn[3];
+    int32_t base[3];
 };
 
-__device__ __forceinline__ Nbhd load_nbhd(const Geometry &g, const Tile &t)
+__device__ __forceinline__ Nbhd load_nbhd(const Geometry &g, int j, int64_t i0)
 {
     Nbhd h;
-    int64_t k = t.i0 >> 6;
-    const uint64_t *base = g.wet + (int64_t)t.j * g.wl + k;  // column j-1, word k-1
+    const int64_t k = i0 >> 6;
+    const uint64_t *wbase = g.wet + (int64_t)j * g.wl + k;  // column j-1, word k-1
+    const int32_t *sbase = g.sb + (int64_t)j * g.wl + k + 1; // column j-1, word k
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const uint64_t *col = base + c * g.wl;
+        const uint64_t *col = wbase + c * g.wl;
         uint64_t w0 = col[0], w1 = col[1], w2 = col[2];
         h.up[c] = (w1 << 1) | (w0 >> 63);
         h.mid[c] = w1;
         h.dn[c] = (w1 >> 1) | (w2 << 63);
+        h.base[c] = sbase[c * g.wl];
     }
     return h;
 }
@@ -142,9 +152,37 @@ __device__ __forceinline__ uint32_t lane_mask(const Nbhd &h, int l)
     return m;
 }
 
-__device__ __forceinline__ uint32_t neighbour_mask(const Geometry &g, const Tile &t)
+// Number of set bits of w below this lane.
+__device__ __forceinline__ uint32_t below(uint64_t w)
 {
-    return lane_mask(load_nbhd(g, t), t.lane);
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(w >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)w, 0u));
+}
+
+// Compact indices of this lane's node and of the source of every
+// direction (meaningful where the node/source is wet).  Row i of column c
+// sits at base[c] + below(mid[c]); row i-1 is the entry before it, row
+// i+1 the entry after it when row i is wet.
+struct Idx {
+    int32_t src[9];
+};
+
+__device__ __forceinline__ Idx lane_idx(const Nbhd &h, int lane)
+{
+    const int32_t l = h.base[0] + (int32_t)below(h.mid[0]);
+    const int32_t c = h.base[1] + (int32_t)below(h.mid[1]);
+    const int32_t r = h.base[2] + (int32_t)below(h.mid[2]);
+    const int32_t wl_ = (int32_t)((h.mid[0] >> lane) & 1), wr_ = (int32_t)((h.mid[2] >> lane) & 1);
+    Idx x;
+    x.src[0] = c;           // own node
+    x.src[1] = l;           // (i, j-1)
+    x.src[2] = c + 1;       // (i+1, j)
+    x.src[3] = r;           // (i, j+1)
+    x.src[4] = c - 1;       // (i-1, j)
+    x.src[5] = l + wl_;     // (i+1, j-1)
+    x.src[6] = r + wr_;     // (i+1, j+1)
+    x.src[7] = r - 1;       // (i-1, j+1)
+    x.src[8] = l - 1;       // (i-1, j-1)
+    return x;
 }
 
 // Element access through a wave-uniform base pointer and a 32-bit byte
@@ -163,84 +201,59 @@ __device__ __forceinline__ void st(T *__restrict__ base, uint32_t byte_off, T v)
     *(T *)((char *)base + byte_off) = v;
 }
 
-// Byte offset of the source of direction d relative to the node (streaming).
-template <typename T>
-__device__ __forceinline__ int32_t src_off(int d, uint32_t L)
-{
-    const int32_t E = (int32_t)sizeof(T), l = (int32_t)L;
-    switch (d) {
-    case 1: return -l;
-    case 2: return E;
-    case 3: return l;
-    case 4: return -E;
-    case 5: return E - l;
-    case 6: return E + l;
-    case 7: return -E + l;
-    case 8: return -E - l;
-    default: return 0;
-    }
-}
-
-// Streams the 9 populations of a node whose sources are all wet.
-template <typename T>
-__device__ __forceinline__ void gather_bulk(const StepArgs &a, int64_t n, T *p)
-{
-    const uint32_t b = (uint32_t)n * (uint32_t)sizeof(T), L = (uint32_t)a.g.ldr * (uint32_t)sizeof(T);
-#pragma unroll
-    for (int d = 0; d < 9; ++d) p[d] = ld((const T *)a.fin_plane[d], b + (uint32_t)src_off<T>(d, L));
-}
-
 // Streams the 9 populations of an interior wet node: one load per
 // direction, its address chosen between the neighbour (stream) and the
 // node's own opposite slot (bounce-back).  Opposite directions are two
 // planes apart, so both candidates are reached from the lower plane of the
 // pair with a non-negative 32-bit offset.
 template <typename T>
-__device__ __forceinline__ void gather_interior(const StepArgs &a, int64_t n, uint32_t m, T *p)
+__device__ __forceinline__ void gather_interior(const StepArgs &a, const Idx &x, uint32_t m, T *p)
 {
-    const uint32_t b = (uint32_t)n * (uint32_t)sizeof(T), L = (uint32_t)a.g.ldr * (uint32_t)sizeof(T);
-    const uint32_t twoP = 2u * (uint32_t)a.g.plane * (uint32_t)sizeof(T);
-    p[0] = ld((const T *)a.fin_plane[0], b);
+    const uint32_t E = (uint32_t)sizeof(T);
+    const uint32_t twoP = 2u * (uint32_t)a.g.plane * E;
+    const uint32_t own = (uint32_t)x.src[0] * E;
+    p[0] = ld((const T *)a.fin_plane[0], own);
 #pragma unroll
     for (int d = 1; d < 9; ++d) {
         const bool wet = (m >> d) & 1;
         const int o = kOpp[d];
         const bool low = d < o;  // d is the lower plane of its pair
         const T *base = (const T *)a.fin_plane[low ? d : o];
-        uint32_t nb = b + (uint32_t)src_off<T>(d, L) + (low ? 0u : twoP);
-        uint32_t ob = b + (low ? twoP : 0u);
+        uint32_t nb = (uint32_t)x.src[d] * E + (low ? 0u : twoP);
+        uint32_t ob = own + (low ? twoP : 0u);
         p[d] = ld(base, wet ? nb : ob);
     }
 }
 
-// Writes the 9 post-collision populations of node n.
+// Same for a perimeter node: sources outside the domain read the frozen
+// slot of the node's edge entry.
+template <typename T>
+__device__ __forceinline__ void gather_edge(const StepArgs &a, const T *__restrict__ frozen, int i, int j,
+                                            const Idx &x, int64_t e, uint32_t m, T *p)
+{
+    const uint32_t E = (uint32_t)sizeof(T);
+    p[0] = ld((const T *)a.fin_plane[0], (uint32_t)x.src[0] * E);
+#pragma unroll
+    for (int d = 1; d < 9; ++d) {
+        int si = i - kOffI[d], sj = j - kOffJ[d];
+        bool out = si < 0 || si >= a.g.nr || sj < 0 || sj >= a.g.nc;
+        if (out) {
+            p[d] = frozen[d * a.g.E + e];
+        } else {
+            bool wet = (m >> d) & 1;
+            p[d] = wet ? ld((const T *)a.fin_plane[d], (uint32_t)x.src[d] * E)
+                       : ld((const T *)a.fin_plane[kOpp[d]], (uint32_t)x.src[0] * E);
+        }
+    }
+}
+
+// Writes the 9 post-collision populations of the node with compact index n.
 template <typename T>
 __device__ __forceinline__ void store9(const StepArgs &a, int64_t n, const T *o)
 {
     const uint32_t b = (uint32_t)n * (uint32_t)sizeof(T);
 #pragma unroll
     for (int d = 0; d < 9; ++d) st((T *)a.fout_plane[d], b, o[d]);
-}
-
-// Same for a perimeter node: sources outside the domain read the frozen
-// slot of the node's edge entry.
-template <typename T>
-__device__ __forceinline__ void gather_edge(const T *__restrict__ F, const T *__restrict__ frozen, const Geometry &g,
-                                            int i, int j, int64_t n, int64_t e, uint32_t m, T *p)
-{
-    p[0] = F[n];
-#pragma unroll
-    for (int d = 1; d < 9; ++d) {
-        int si = i - kOffI[d], sj = j - kOffJ[d];
-        bool out = si < 0 || si >= g.nr || sj < 0 || sj >= g.nc;
-        if (out) {
-            p[d] = frozen[d * g.E + e];
-        } else {
-            bool wet = (m >> d) & 1;
-            int64_t idx = wet ? d * g.plane + (int64_t)sj * g.ldr + si : kOpp[d] * g.plane + n;
-            p[d] = F[idx];
-        }
-    }
 }
 
 // ---------------------------------------------------------------- FP64 --
@@ -351,22 +364,22 @@ __device__ __forceinline__ void zouhe_pressure64(int bt, double *p, double rho)
 // Pre-collision populations of wet node (i, j) in FP64, with boundary
 // handling.  Returns the open-boundary policy of the node (-1 = fluid).
 template <bool GATHER>
-__device__ __forceinline__ int load_p64(const StepArgs &a, const double *__restrict__ F, int i, int j, int64_t n,
-                                        uint32_t m, bool edge, int64_t &e, double *p)
+__device__ __forceinline__ int load_p64(const StepArgs &a, int i, int j, const Idx &x, uint32_t m, bool edge,
+                                        int64_t &e, double *p)
 {
     int policy = -1;
     if (!edge) {
-        if (GATHER) gather_interior(a, n, m, p);
+        if (GATHER) gather_interior(a, x, m, p);
         else
 #pragma unroll
-            for (int d = 0; d < 9; ++d) p[d] = F[d * a.g.plane + n];
+            for (int d = 0; d < 9; ++d) p[d] = ld((const double *)a.fin_plane[d], (uint32_t)x.src[0] * 8u);
         return -1;
     }
     e = edge_slot(i, j, a.g.nr, a.g.nc);
-    if (GATHER) gather_edge(F, (const double *)a.frozen, a.g, i, j, n, e, m, p);
+    if (GATHER) gather_edge(a, (const double *)a.frozen, i, j, x, e, m, p);
     else
 #pragma unroll
-        for (int d = 0; d < 9; ++d) p[d] = F[d * a.g.plane + n];
+        for (int d = 0; d < 9; ++d) p[d] = ld((const double *)a.fin_plane[d], (uint32_t)x.src[0] * 8u);
     int kind = a.g.edge_kind[e];
     if (kind != kEdgePlain) {
         policy = kind == kEdgeInlet ? a.inlet_policy : a.outlet_policy;
@@ -427,17 +440,16 @@ __device__ __forceinline__ void collide_store_f64(const StepArgs &a, const doubl
 
 // One node of any kind (perimeter, open boundary, bounce-back).
 template <bool GATHER>
-__device__ __forceinline__ void node_general_f64(const StepArgs &a, const Tile &t, const Nbhd &h)
+__device__ __forceinline__ void node_general_f64(const StepArgs &a, int i, int j, int lane, const Nbhd &h)
 {
-    uint32_t m = lane_mask(h, t.lane);
+    uint32_t m = lane_mask(h, lane);
     if (!(m & 1)) return;
-    const int i = t.i, j = t.j;
-    const int64_t n = (int64_t)j * a.g.ldr + i;
     const bool edge = (i == 0) | (i == a.g.nr - 1) | (j == 0) | (j == a.g.nc - 1);
+    const Idx x = lane_idx(h, lane);
     double p[9];
     int64_t e = 0;
-    int policy = load_p64<GATHER>(a, (const double *)a.fin, i, j, n, m, edge, e, p);
-    collide_store_f64(a, p, policy, e, i, j, n);
+    int policy = load_p64<GATHER>(a, i, j, x, m, edge, e, p);
+    collide_store_f64(a, p, policy, e, i, j, x.src[0]);
 }
 
 // ------------------------------------------------------- shifted FP32 --
@@ -527,25 +539,26 @@ __device__ __forceinline__ void zouhe_pressure32(int bt, float *g, float rho0, f
 }
 
 template <typename T, bool GATHER>
-__device__ __forceinline__ int load_g(const StepArgs &a, const T *__restrict__ F, int i, int j, int64_t n, uint32_t m,
-                                      bool edge, int64_t &e, float *g)
+__device__ __forceinline__ int load_g(const StepArgs &a, int i, int j, const Idx &x, uint32_t m, bool edge,
+                                      int64_t &e, float *g)
 {
     T raw[9];
     int policy = -1;
+    const uint32_t own = (uint32_t)x.src[0] * (uint32_t)sizeof(T);
     if (!edge) {
-        if (GATHER) gather_interior(a, n, m, raw);
+        if (GATHER) gather_interior(a, x, m, raw);
         else
 #pragma unroll
-            for (int d = 0; d < 9; ++d) raw[d] = F[d * a.g.plane + n];
+            for (int d = 0; d < 9; ++d) raw[d] = ld((const T *)a.fin_plane[d], own);
 #pragma unroll
         for (int d = 0; d < 9; ++d) g[d] = to_g<T>(raw[d], a.inv_scale);
         return -1;
     }
     e = edge_slot(i, j, a.g.nr, a.g.nc);
-    if (GATHER) gather_edge(F, (const T *)a.frozen, a.g, i, j, n, e, m, raw);
+    if (GATHER) gather_edge(a, (const T *)a.frozen, i, j, x, e, m, raw);
     else
 #pragma unroll
-        for (int d = 0; d < 9; ++d) raw[d] = F[d * a.g.plane + n];
+        for (int d = 0; d < 9; ++d) raw[d] = ld((const T *)a.fin_plane[d], own);
 #pragma unroll
     for (int d = 0; d < 9; ++d) g[d] = to_g<T>(raw[d], a.inv_scale);
     int kind = a.g.edge_kind[e];
@@ -657,17 +670,16 @@ __device__ __forceinline__ void collide_store_shifted(const StepArgs &a, const f
 }
 
 template <typename T, bool GATHER>
-__device__ __forceinline__ void node_general_shifted(const StepArgs &a, const Tile &t, const Nbhd &h)
+__device__ __forceinline__ void node_general_shifted(const StepArgs &a, int i, int j, int lane, const Nbhd &h)
 {
-    uint32_t m = lane_mask(h, t.lane);
+    uint32_t m = lane_mask(h, lane);
     if (!(m & 1)) return;
-    const int i = t.i, j = t.j;
-    const int64_t n = (int64_t)j * a.g.ldr + i;
     const bool edge = (i == 0) | (i == a.g.nr - 1) | (j == 0) | (j == a.g.nc - 1);
+    const Idx x = lane_idx(h, lane);
     float g[9];
     int64_t e = 0;
-    int policy = load_g<T, GATHER>(a, (const T *)a.fin, i, j, n, m, edge, e, g);
-    collide_store_shifted<T>(a, g, policy, e, i, j, n);
+    int policy = load_g<T, GATHER>(a, i, j, x, m, edge, e, g);
+    collide_store_shifted<T>(a, g, policy, e, i, j, x.src[0]);
 }
 
 // ------------------------------------------------------------ step kernel --
@@ -698,65 +710,50 @@ __global__ __launch_bounds__(kBlockRows) void step_kernel(StepArgs a)
     if (perim || !GATHER) {
 #pragma unroll 1
         for (int r = 0; r < R; ++r) {
-            Tile t;
-            t.j = j;
-            t.i0 = row0 + 64 * r;
-            t.lane = lane;
-            t.i = (int)t.i0 + lane;
-            if (t.i0 >= a.g.ldr) break;
-            const Nbhd h = load_nbhd(a.g, t);
-            if constexpr (sizeof(T) == 8) node_general_f64<GATHER>(a, t, h);
-            else node_general_shifted<T, GATHER>(a, t, h);
+            const int64_t i0 = row0 + 64 * r;
+            if (i0 >= a.g.ldr) break;
+            const Nbhd h = load_nbhd(a.g, j, i0);
+            if (h.mid[1] == 0) continue;
+            const int i = (int)i0 + lane;
+            if constexpr (sizeof(T) == 8) node_general_f64<GATHER>(a, i, j, lane, h);
+            else node_general_shifted<T, GATHER>(a, i, j, lane, h);
         }
         return;
     }
     V p[R][9];
-    bool act[R], any[R];
+    bool act[R];
+    int32_t own[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        Tile t;
-        t.j = j;
-        t.i0 = row0 + 64 * r;
-        t.lane = lane;
-        t.i = (int)t.i0 + lane;
+        const int64_t i0 = row0 + 64 * r;
         act[r] = false;
-        any[r] = false;
-        if (t.i0 < a.g.ldr) {
-            const Nbhd h = load_nbhd(a.g, t);
-            const int64_t n = (int64_t)j * a.g.ldr + t.i;
-            any[r] = h.mid[1] != 0;
-            T raw[9];
-            if (wave_is_bulk(h)) {
-                gather_bulk(a, n, raw);
-                act[r] = true;
-            } else {
-                uint32_t m = lane_mask(h, lane);
+        own[r] = 0;
+        if (i0 < a.g.ldr) {
+            const Nbhd h = load_nbhd(a.g, j, i0);
+            if (h.mid[1] != 0) {
+                const Idx x = lane_idx(h, lane);
+                own[r] = x.src[0];
+                // A segment whose 3x3 neighbourhood is all wet (a scalar,
+                // wave-uniform test) streams every direction; otherwise the
+                // per-lane mask selects bounce-back sources.
+                const uint32_t m = wave_is_bulk(h) ? 0x1FFu : lane_mask(h, lane);
                 act[r] = m & 1;
-                if (act[r]) gather_interior(a, n, m, raw);
-            }
+                T raw[9] = {};
+                if (act[r]) gather_interior(a, x, m, raw);
 #pragma unroll
-            for (int d = 0; d < 9; ++d) {
-                if constexpr (sizeof(T) == 8) p[r][d] = raw[d];
-                else p[r][d] = to_g<T>(raw[d], a.inv_scale);
+                for (int d = 0; d < 9; ++d) {
+                    if constexpr (sizeof(T) == 8) p[r][d] = raw[d];
+                    else p[r][d] = to_g<T>(raw[d], a.inv_scale);
+                }
             }
         }
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
+        if (!act[r]) continue;
         const int i = (int)row0 + 64 * r + lane;
-        const int64_t n = (int64_t)j * a.g.ldr + i;
-        if (act[r]) {
-            if constexpr (sizeof(T) == 8) collide_store_f64(a, p[r], -1, 0, i, j, n);
-            else collide_store_shifted<T>(a, p[r], -1, 0, i, j, n);
-        } else if (any[r] && i < a.g.nr) {
-            // Dry node in a wave with wet nodes: write a placeholder so the
-            // wave's stores cover whole cache lines (a partly written line
-            // costs HBM a read-modify-write).  Dry slots are never read.
-            T zero[9];
-#pragma unroll
-            for (int d = 0; d < 9; ++d) zero[d] = T(0);
-            store9(a, n, zero);
-        }
+        if constexpr (sizeof(T) == 8) collide_store_f64(a, p[r], -1, 0, i, j, own[r]);
+        else collide_store_shifted<T>(a, p[r], -1, 0, i, j, own[r]);
     }
 }
 
@@ -801,21 +798,22 @@ __global__ __launch_bounds__(kBlockRows) void readout(StepArgs a, double *out, d
     Tile t = tile_of(a);
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (t.valid) {
-        uint32_t m = neighbour_mask(a.g, t);
+        const Nbhd h = load_nbhd(a.g, t.j, t.i0);
+        uint32_t m = lane_mask(h, t.lane);
         const int i = t.i, j = t.j;
         const int64_t N = (int64_t)a.g.nr * a.g.nc;
         const int64_t q = (int64_t)j * a.g.nr + i;
         if (m & 1) {
-            const int64_t n = (int64_t)j * a.g.ldr + i;
             const bool edge = (i == 0) | (i == a.g.nr - 1) | (j == 0) | (j == a.g.nc - 1);
+            const Idx x = lane_idx(h, t.lane);
             double f[9];
             int64_t e = 0;
             int policy;
             if constexpr (sizeof(T) == 8) {
-                policy = load_p64<GATHER>(a, (const double *)a.fin, i, j, n, m, edge, e, f);
+                policy = load_p64<GATHER>(a, i, j, x, m, edge, e, f);
             } else {
                 float g[9];
-                policy = load_g<T, GATHER>(a, (const T *)a.fin, i, j, n, m, edge, e, g);
+                policy = load_g<T, GATHER>(a, i, j, x, m, edge, e, g);
 #pragma unroll
                 for (int d = 0; d < 9; ++d) f[d] = (double)g[d] + kW[d];
             }
@@ -842,15 +840,15 @@ __global__ __launch_bounds__(kBlockRows) void readout(StepArgs a, double *out, d
 // Converts a reference-layout FP64 state to the storage format on the wet
 // nodes and fills the frozen slots of the perimeter.
 template <typename T>
-__global__ __launch_bounds__(kBlockRows) void upload(StepArgs a, const double *src, T *dst, T *frozen)
+__global__ __launch_bounds__(kBlockRows) void upload(StepArgs a, const double *src, T *frozen)
 {
     Tile t = tile_of(a);
     if (!t.valid) return;
-    uint32_t m = neighbour_mask(a.g, t);
-    if (!(m & 1)) return;
+    const Nbhd h = load_nbhd(a.g, t.j, t.i0);
+    if (!((h.mid[1] >> t.lane) & 1)) return;
     const int i = t.i, j = t.j;
     const int64_t N = (int64_t)a.g.nr * a.g.nc, q = (int64_t)j * a.g.nr + i;
-    const int64_t n = (int64_t)j * a.g.ldr + i;
+    const uint32_t off = (uint32_t)lane_idx(h, t.lane).src[0] * (uint32_t)sizeof(T);
     const bool edge = (i == 0) | (i == a.g.nr - 1) | (j == 0) | (j == a.g.nc - 1);
     const int64_t e = edge ? edge_slot(i, j, a.g.nr, a.g.nc) : 0;
 #pragma unroll
@@ -859,7 +857,7 @@ __global__ __launch_bounds__(kBlockRows) void upload(StepArgs a, const double *s
         T v;
         if constexpr (sizeof(T) == 8) v = f;
         else v = from_g<T>((float)(f - kW[d]), a.scale);
-        dst[d * a.g.plane + n] = v;
+        st((T *)a.fout_plane[d], off, v);
         if (edge) frozen[d * a.g.E + e] = v;
     }
 }
@@ -867,7 +865,7 @@ __global__ __launch_bounds__(kBlockRows) void upload(StepArgs a, const double *s
 // Lattice.hpp:294-328: feq(1, 0) = w_d on the wet nodes (0 in shifted
 // storage), rho = 1 and u = 0 everywhere.
 template <typename T>
-__global__ __launch_bounds__(kBlockRows) void init_eq(StepArgs a, T *dst, T *frozen, double *macro)
+__global__ __launch_bounds__(kBlockRows) void init_eq(StepArgs a, T *frozen, double *macro)
 {
     Tile t = tile_of(a);
     if (!t.valid) return;
@@ -878,9 +876,9 @@ __global__ __launch_bounds__(kBlockRows) void init_eq(StepArgs a, T *dst, T *fro
         macro[N + q] = 0.0;
         macro[2 * N + q] = 0.0;
     }
-    uint32_t m = neighbour_mask(a.g, t);
-    if (!(m & 1)) return;
-    const int64_t n = (int64_t)j * a.g.ldr + i;
+    const Nbhd h = load_nbhd(a.g, t.j, t.i0);
+    if (!((h.mid[1] >> t.lane) & 1)) return;
+    const uint32_t off = (uint32_t)lane_idx(h, t.lane).src[0] * (uint32_t)sizeof(T);
     const bool edge = (i == 0) | (i == a.g.nr - 1) | (j == 0) | (j == a.g.nc - 1);
     const int64_t e = edge ? edge_slot(i, j, a.g.nr, a.g.nc) : 0;
     double e9[9];
@@ -890,24 +888,33 @@ __global__ __launch_bounds__(kBlockRows) void init_eq(StepArgs a, T *dst, T *fro
         T v;
         if constexpr (sizeof(T) == 8) v = e9[d];
         else v = from_g<T>((float)(e9[d] - kW[d]), a.scale);
-        dst[d * a.g.plane + n] = v;
+        st((T *)a.fout_plane[d], off, v);
         if (edge) frozen[d * a.g.E + e] = v;
     }
 }
 
-template <typename T>
-__device__ __forceinline__ double fstar_at(const StepArgs &a, const T *F, int64_t i, int64_t j, int d)
+// Compact index of wet node (i, j), any thread.
+__device__ __forceinline__ int64_t cidx_of(const Geometry &g, int64_t i, int64_t j)
 {
-    T v = F[d * a.g.plane + j * a.g.ldr + i];
+    const int64_t w = (j + 1) * g.wl + (i >> 6) + 1;
+    const uint64_t word = g.wet[w];
+    const uint64_t lowbits = (i & 63) ? (word & ((1ull << (i & 63)) - 1)) : 0ull;
+    return (int64_t)g.sb[w] + __popcll(lowbits);
+}
+
+template <typename T>
+__device__ __forceinline__ double fstar_at(const StepArgs &a, int64_t i, int64_t j, int d)
+{
+    const T *F = (const T *)a.fin_plane[d];
+    T v = F[cidx_of(a.g, i, j)];
     if constexpr (sizeof(T) == 8) return (double)v;
     else return (double)to_g<T>(v, a.inv_scale) + kW[d];
 }
 
 // FlowAnalyzer.hpp:386-430 for one point per thread, then a block sum.
 template <typename T>
-__global__ __launch_bounds__(kBlockRows) void lift_drag(StepArgs a, const T *F, const int64_t *ij,
-                                                        const int32_t *masks, int64_t n, double *per_point,
-                                                        double *partials)
+__global__ __launch_bounds__(kBlockRows) void lift_drag(StepArgs a, const int64_t *ij, const int32_t *masks,
+                                                        int64_t n, double *per_point, double *partials)
 {
     int64_t q = (int64_t)blockIdx.x * kBlockRows + threadIdx.x;
     double acc[2] = {0.0, 0.0};
@@ -916,27 +923,27 @@ __global__ __launch_bounds__(kBlockRows) void lift_drag(StepArgs a, const T *F, 
         const int32_t msk = masks[q];
         const double h = sqrt(2.0);
         double local_lift = 0., local_drag = 0.;
-        if (msk & (1 << 1)) local_drag -= fstar_at(a, F, i, j + 1, 3);
-        if (msk & (1 << 2)) local_lift -= fstar_at(a, F, i - 1, j, 4);
-        if (msk & (1 << 3)) local_drag += fstar_at(a, F, i, j - 1, 1);
-        if (msk & (1 << 4)) local_lift += fstar_at(a, F, i + 1, j, 2);
+        if (msk & (1 << 1)) local_drag -= fstar_at<T>(a, i, j + 1, 3);
+        if (msk & (1 << 2)) local_lift -= fstar_at<T>(a, i - 1, j, 4);
+        if (msk & (1 << 3)) local_drag += fstar_at<T>(a, i, j - 1, 1);
+        if (msk & (1 << 4)) local_lift += fstar_at<T>(a, i + 1, j, 2);
         if (msk & (1 << 5)) {
-            double v = fstar_at(a, F, i - 1, j + 1, 7);
+            double v = fstar_at<T>(a, i - 1, j + 1, 7);
             local_drag -= (v) * h / 2.0;
             local_lift -= (v) * h / 2.0;
         }
         if (msk & (1 << 6)) {
-            double v = fstar_at(a, F, i - 1, j - 1, 8);
+            double v = fstar_at<T>(a, i - 1, j - 1, 8);
             local_drag += (v) * h / 2.0;
             local_lift -= (v) * h / 2.0;
         }
         if (msk & (1 << 7)) {
-            double v = fstar_at(a, F, i + 1, j - 1, 5);
+            double v = fstar_at<T>(a, i + 1, j - 1, 5);
             local_drag += (v) * h / 2.0;
             local_lift += (v) * h / 2.0;
         }
         if (msk & (1 << 8)) {
-            double v = fstar_at(a, F, i + 1, j + 1, 6);
+            double v = fstar_at<T>(a, i + 1, j + 1, 6);
             local_drag -= (v) * h / 2.0;
             local_lift += (v) * h / 2.0;
         }
@@ -967,7 +974,6 @@ __global__ __launch_bounds__(kBlockRows) void reduce_partials(const double *part
 
 int64_t blocks_of(const StepArgs &a) { return (int64_t)a.segs * a.g.nc; }
 
-
 template <typename K>
 hipError_t launch(K kernel, int64_t nb, hipStream_t s, StepArgs a)
 {
@@ -975,22 +981,6 @@ hipError_t launch(K kernel, int64_t nb, hipStream_t s, StepArgs a)
     set_planes(a);
     hipLaunchKernelGGL(kernel, dim3((unsigned)nb), dim3(kBlockRows), 0, s, a);
     return hipGetLastError();
-}
-
-}  // namespace
-
-void set_planes(StepArgs &a)
-{
-    const size_t stride = (size_t)a.g.plane * (size_t)a.esize;
-    for (int d = 0; d < 9; ++d) {
-        a.fin_plane[d] = a.fin ? (const char *)a.fin + (size_t)d * stride : nullptr;
-        a.fout_plane[d] = a.fout ? (char *)a.fout + (size_t)d * stride : nullptr;
-    }
-}
-
-int32_t step_segments(int32_t storage)
-{
-    return storage == kF64 ? 2 : 4;
 }
 
 int64_t step_blocks(const StepArgs &a, int32_t R)
@@ -1006,6 +996,17 @@ hipError_t launch_step_t(bool gather, StepArgs a, hipStream_t s)
     a.wpc = (int32_t)((a.g.ldr / 64 + R - 1) / R);
     const int64_t nb = step_blocks(a, R);
     return gather ? launch(step_kernel<T, true, R>, nb, s, a) : launch(step_kernel<T, false, R>, nb, s, a);
+}
+
+}  // namespace
+
+void set_planes(StepArgs &a)
+{
+    const size_t stride = (size_t)a.g.plane * (size_t)a.esize;
+    for (int d = 0; d < 9; ++d) {
+        a.fin_plane[d] = a.fin ? (const char *)a.fin + (size_t)d * stride : nullptr;
+        a.fout_plane[d] = a.fout ? (char *)a.fout + (size_t)d * stride : nullptr;
+    }
 }
 
 hipError_t launch_step(int32_t storage, bool gather, const StepArgs &a, hipStream_t s)
@@ -1048,13 +1049,14 @@ hipError_t launch_upload(int32_t storage, const StepArgs &a0, const double *src,
 {
     StepArgs a = a0;
     a.nblocks = blocks_of(a);
+    a.fin = nullptr;
+    a.fout = dst;
+    set_planes(a);
     dim3 grid((unsigned)a.nblocks), block(kBlockRows);
     switch (storage) {
-    case kF64: hipLaunchKernelGGL(upload<double>, grid, block, 0, s, a, src, (double *)dst, (double *)frozen); break;
-    case kF32: hipLaunchKernelGGL(upload<float>, grid, block, 0, s, a, src, (float *)dst, (float *)frozen); break;
-    default:
-        hipLaunchKernelGGL(upload<_Float16>, grid, block, 0, s, a, src, (_Float16 *)dst, (_Float16 *)frozen);
-        break;
+    case kF64: hipLaunchKernelGGL(upload<double>, grid, block, 0, s, a, src, (double *)frozen); break;
+    case kF32: hipLaunchKernelGGL(upload<float>, grid, block, 0, s, a, src, (float *)frozen); break;
+    default: hipLaunchKernelGGL(upload<_Float16>, grid, block, 0, s, a, src, (_Float16 *)frozen); break;
     }
     return hipGetLastError();
 }
@@ -1064,38 +1066,34 @@ hipError_t launch_init_eq(int32_t storage, const StepArgs &a0, void *dst, void *
 {
     StepArgs a = a0;
     a.nblocks = blocks_of(a);
+    a.fin = nullptr;
+    a.fout = dst;
+    set_planes(a);
     dim3 grid((unsigned)a.nblocks), block(kBlockRows);
     switch (storage) {
-    case kF64: hipLaunchKernelGGL(init_eq<double>, grid, block, 0, s, a, (double *)dst, (double *)frozen, macro); break;
-    case kF32: hipLaunchKernelGGL(init_eq<float>, grid, block, 0, s, a, (float *)dst, (float *)frozen, macro); break;
-    default:
-        hipLaunchKernelGGL(init_eq<_Float16>, grid, block, 0, s, a, (_Float16 *)dst, (_Float16 *)frozen, macro);
-        break;
+    case kF64: hipLaunchKernelGGL(init_eq<double>, grid, block, 0, s, a, (double *)frozen, macro); break;
+    case kF32: hipLaunchKernelGGL(init_eq<float>, grid, block, 0, s, a, (float *)frozen, macro); break;
+    default: hipLaunchKernelGGL(init_eq<_Float16>, grid, block, 0, s, a, (_Float16 *)frozen, macro); break;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_lift_drag(int32_t storage, const StepArgs &a, const void *fstar, const int64_t *ij,
+hipError_t launch_lift_drag(int32_t storage, const StepArgs &a0, const void *fstar, const int64_t *ij,
                             const int32_t *masks, int64_t n, double *per_point, double *partials,
                             int64_t *nblocks_out, hipStream_t s)
 {
+    StepArgs a = a0;
+    a.fin = fstar;
+    a.fout = nullptr;
+    set_planes(a);
     int64_t nb = (n + kBlockRows - 1) / kBlockRows;
     if (nb < 1) nb = 1;
     *nblocks_out = nb;
     dim3 grid((unsigned)nb), block(kBlockRows);
     switch (storage) {
-    case kF64:
-        hipLaunchKernelGGL(lift_drag<double>, grid, block, 0, s, a, (const double *)fstar, ij, masks, n, per_point,
-                           partials);
-        break;
-    case kF32:
-        hipLaunchKernelGGL(lift_drag<float>, grid, block, 0, s, a, (const float *)fstar, ij, masks, n, per_point,
-                           partials);
-        break;
-    default:
-        hipLaunchKernelGGL(lift_drag<_Float16>, grid, block, 0, s, a, (const _Float16 *)fstar, ij, masks, n,
-                           per_point, partials);
-        break;
+    case kF64: hipLaunchKernelGGL(lift_drag<double>, grid, block, 0, s, a, ij, masks, n, per_point, partials); break;
+    case kF32: hipLaunchKernelGGL(lift_drag<float>, grid, block, 0, s, a, ij, masks, n, per_point, partials); break;
+    default: hipLaunchKernelGGL(lift_drag<_Float16>, grid, block, 0, s, a, ij, masks, n, per_point, partials); break;
     }
     return hipGetLastError();
 }

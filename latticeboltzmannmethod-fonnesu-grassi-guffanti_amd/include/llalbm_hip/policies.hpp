@@ -1,0 +1,276 @@
+// HIP-tagged policy classes and the HIPPolicy parallelization policy.
+//
+// The reference selects its numerics at compile time: a
+// LatticeConfiguration bundles seven policy types
+// (llalbm/src/core/lattices/LatticeConfiguration.hpp:32-60), all carrying the
+// same parallelization tag, and a parallelization policy such as
+// OMPPolicy<2, Configuration> (parallelization/OMPPolicy.hpp:36-164)
+// dispatches Lattice::perform_lbm's calls to them.  This header adds the
+// HIP tag, the IsGloballyHIP concept (PolicyTypes.hpp:72-125 pattern) and
+// HIP-tagged policy classes with the same static parameter setters, so a
+// reference configuration switches to the GPU by renaming its policies.
+// The numerics themselves run in the C ABI (lbm_amd.h); these classes hold
+// parameters and translate them into an lbm_desc.
+#ifndef LLALBM_HIP_POLICIES_HPP
+#define LLALBM_HIP_POLICIES_HPP
+
+#include <array>
+#include <concepts>
+#include <functional>
+#include <iostream>
+#include <type_traits>
+
+#include "../../../include/lbm_amd.h"
+#include "aliases.hpp"
+
+namespace llalbm::core
+{
+using util::BoundaryPoint;
+using util::ObstaclePoint;
+using util::Point;
+
+// ----- kind and parallelization tags (PolicyTypes.hpp:36-45) -----
+class CollisionPolicyTag {};
+class BoundaryPolicyTag {};
+class InitializationPolicyTag {};
+class HIPTag {};
+
+template <typename T>
+concept IsCollisionPolicy = std::is_base_of_v<CollisionPolicyTag, T>;
+template <typename T>
+concept IsBoundaryPolicy = std::is_base_of_v<BoundaryPolicyTag, T>;
+template <typename T>
+concept IsInitializationPolicy = std::is_base_of_v<InitializationPolicyTag, T>;
+template <typename T>
+concept IsHIP = std::is_base_of_v<HIPTag, T>;
+
+// ----- LatticeConfiguration (LatticeConfiguration.hpp:32-60) -----
+template <std::size_t dim, typename CollisionPolicy, typename WallPolicy, typename ObstaclePolicy,
+          typename InletPolicy, typename OutletPolicy, typename InitializationPolicy, typename EquilibriumPolicy>
+    requires IsCollisionPolicy<CollisionPolicy> && IsBoundaryPolicy<WallPolicy> && IsBoundaryPolicy<ObstaclePolicy> &&
+             IsBoundaryPolicy<InletPolicy> && IsBoundaryPolicy<OutletPolicy> &&
+             IsInitializationPolicy<InitializationPolicy>
+class LatticeConfiguration
+{
+public:
+    static constexpr std::size_t dimensions = dim;
+    using collision_policy_t = CollisionPolicy;
+    using wall_policy_t = WallPolicy;
+    using obstacle_policy_t = ObstaclePolicy;
+    using inlet_policy_t = InletPolicy;
+    using outlet_policy_t = OutletPolicy;
+    using initialization_policy_t = InitializationPolicy;
+    using equilibrium_policy_t = EquilibriumPolicy;
+};
+
+template <typename Configuration>
+concept IsGloballyHIP = std::conjunction_v<std::is_base_of<HIPTag, typename Configuration::collision_policy_t>,
+                                           std::is_base_of<HIPTag, typename Configuration::wall_policy_t>,
+                                           std::is_base_of<HIPTag, typename Configuration::obstacle_policy_t>,
+                                           std::is_base_of<HIPTag, typename Configuration::inlet_policy_t>,
+                                           std::is_base_of<HIPTag, typename Configuration::outlet_policy_t>,
+                                           std::is_base_of<HIPTag, typename Configuration::initialization_policy_t>,
+                                           std::is_base_of<HIPTag, typename Configuration::equilibrium_policy_t>>;
+
+namespace collisions
+{
+template <std::size_t dim>
+class HIPBGKCollisionPolicy;
+
+/// BGK collision on the GPU; parameters as BGKCollisionPolicy<2>
+/// (BGKCollision.hpp:67-107).
+template <>
+class HIPBGKCollisionPolicy<2> : public CollisionPolicyTag, public HIPTag
+{
+public:
+    static inline double tau = 0.0;
+    static constexpr int32_t kind = LBM_BGK;
+
+    static void initialize(const double &tau_, const double &max_speed = 0.2)
+    {
+        tau = tau_;
+        std::cout << "Inizialization done:" << std::endl;
+        std::cout << " Tau: " << tau << std::endl;
+        std::cout << " Max Speed: " << max_speed << std::endl;
+        if (tau <= 0.55) std::cout << " The simulation might have stability issues." << std::endl;
+    }
+
+    static void allowed_tau(const double &max_speed, const double &reynolds, const double &cs2 = 1.0 / 3.0)
+    {
+        tau = 0.5 + max_speed / (cs2 * reynolds);
+    }
+
+    static void fill(lbm_desc &d) { d.collision = LBM_BGK; d.tau = tau; }
+};
+
+template <std::size_t dim>
+class HIPTRTCollisionPolicy;
+
+/// TRT collision on the GPU; parameters as TRTCollisionPolicy<2>
+/// (TRTCollision.hpp:69-134).
+template <>
+class HIPTRTCollisionPolicy<2> : public CollisionPolicyTag, public HIPTag
+{
+public:
+    static inline double tau_even = 0.0;
+    static inline double tau_odd = 0.0;
+    static inline double sound_velocity = 0.0;
+    static constexpr int32_t kind = LBM_TRT;
+
+    static void initialize(const double &tau_even_, const double &tau_odd_, const double &sound_velocity_)
+    {
+        tau_even = tau_even_;
+        tau_odd = tau_odd_;
+        sound_velocity = sound_velocity_;
+    }
+    static double compute_kinematic_viscosity() { return (tau_even - 0.5) * sound_velocity * sound_velocity; }
+    static double compute_magic_parameter() { return (tau_even - 0.5) * (tau_odd - 0.5); }
+    static void enforce_magic_parameter(const double &magic_parameter = 1.0 / 4.0)
+    {
+        tau_odd = (magic_parameter) * (1.0 / (tau_even - 0.5)) + 0.5;
+    }
+
+    static void fill(lbm_desc &d) { d.collision = LBM_TRT; d.tau_even = tau_even; d.tau_odd = tau_odd; }
+};
+} // namespace collisions
+
+namespace boundaries
+{
+template <std::size_t dim>
+class HIPBounceBackPolicy;
+/// Half-way bounce-back on walls and wet obstacles (BounceBackPolicy.hpp:68-187).
+template <>
+class HIPBounceBackPolicy<2> : public BoundaryPolicyTag, public HIPTag
+{
+};
+
+template <std::size_t dim>
+class HIPZouHePolicy;
+/// Zou-He velocity boundary on an inlet/outlet list (ZouHePolicy.hpp:58-123).
+template <>
+class HIPZouHePolicy<2> : public BoundaryPolicyTag, public HIPTag
+{
+public:
+    static constexpr int32_t open_policy = LBM_ZOUHE_VELOCITY;
+};
+
+template <std::size_t dim>
+class HIPZouHePressurePolicy;
+/// Zou-He pressure boundary (no reference counterpart): density prescribed
+/// by the initializer's density functions.
+template <>
+class HIPZouHePressurePolicy<2> : public BoundaryPolicyTag, public HIPTag
+{
+public:
+    static constexpr int32_t open_policy = LBM_ZOUHE_PRESSURE;
+};
+
+template <std::size_t dim>
+class HIPOpenBoundaryPolicy;
+/// No-op open boundary (OpenBoundaryPolicy.hpp:58-61).
+template <>
+class HIPOpenBoundaryPolicy<2> : public BoundaryPolicyTag, public HIPTag
+{
+public:
+    static constexpr int32_t open_policy = LBM_OPEN_NOOP;
+};
+} // namespace boundaries
+
+namespace initializers
+{
+template <std::size_t dim>
+class HIPVelocityInitializer;
+
+/// Prescribed inlet/outlet values, evaluated on the host once per step
+/// (VelocityInitializer<2>, InletOutletInitializer.hpp:34-156).  The
+/// density functions feed HIPZouHePressurePolicy lists.
+template <>
+class HIPVelocityInitializer<2> : public InitializationPolicyTag, public HIPTag
+{
+public:
+    using Fn = std::function<double(double, BoundaryPoint<2>)>;
+    static inline std::array<Fn, 2> inlet_update_function{};
+    static inline std::array<Fn, 2> outlet_update_function{};
+    static inline Fn inlet_density_function{};
+    static inline Fn outlet_density_function{};
+
+    static void attach_update_functions(std::array<Fn, 2> &inlet_fns, std::array<Fn, 2> &outlet_fns)
+    {
+        inlet_update_function = inlet_fns;
+        outlet_update_function = outlet_fns;
+    }
+    static void attach_density_functions(const Fn &inlet_rho, const Fn &outlet_rho)
+    {
+        inlet_density_function = inlet_rho;
+        outlet_density_function = outlet_rho;
+    }
+};
+} // namespace initializers
+
+namespace equilibrium
+{
+template <std::size_t dim>
+class HIPDefaultEquilibrium;
+/// Second-order D2Q9 equilibrium (DefaultEquilibrium.hpp:54-155).
+template <>
+class HIPDefaultEquilibrium<2> : public HIPTag
+{
+};
+} // namespace equilibrium
+
+template <std::size_t dim, typename Configuration>
+class HIPPolicy;
+
+/// GPU parallelization policy.  Where OMPPolicy forwards each of
+/// perform_lbm's calls to a policy class operating on host tensors, this
+/// policy describes the whole configuration to the C ABI once
+/// (make_desc) and evaluates the inlet/outlet functions per step
+/// (open_values); Lattice<HIPPolicy<2, C>> then runs the fused step.
+template <typename Configuration>
+    requires IsGloballyHIP<Configuration>
+class HIPPolicy<2, Configuration> : public HIPTag
+{
+public:
+    static constexpr std::size_t dims = Configuration::dimensions;
+    using CollisionPolicy = typename Configuration::collision_policy_t;
+    using InletPolicy = typename Configuration::inlet_policy_t;
+    using OutletPolicy = typename Configuration::outlet_policy_t;
+    using InitializationPolicy = typename Configuration::initialization_policy_t;
+
+    static lbm_desc make_desc(util::Index nr, util::Index nc, int32_t storage = LBM_STORE_F64, int32_t device = 0)
+    {
+        lbm_desc d{};
+        d.n_rows = nr;
+        d.n_cols = nc;
+        d.storage = storage;
+        d.tau = d.tau_even = d.tau_odd = 1.0;
+        CollisionPolicy::fill(d);
+        d.inlet_policy = InletPolicy::open_policy;
+        d.outlet_policy = OutletPolicy::open_policy;
+        d.device = device;
+        return d;
+    }
+
+    /// Values of the update functions at time t for the inlet (which = 0)
+    /// or outlet (1) list (InletOutletInitializer.hpp:90-129).
+    static void open_values(double t, int which, const std::vector<BoundaryPoint<2>> &nodes, std::vector<double> &ux,
+                            std::vector<double> &uy, std::vector<double> &rho)
+    {
+        const auto &fns = which == 0 ? InitializationPolicy::inlet_update_function
+                                     : InitializationPolicy::outlet_update_function;
+        const auto &rfn = which == 0 ? InitializationPolicy::inlet_density_function
+                                     : InitializationPolicy::outlet_density_function;
+        ux.resize(nodes.size());
+        uy.resize(nodes.size());
+        rho.resize(nodes.size());
+        for (std::size_t q = 0; q < nodes.size(); ++q) {
+            ux[q] = fns[0] ? fns[0](t, nodes[q]) : 0.0;
+            uy[q] = fns[1] ? fns[1](t, nodes[q]) : 0.0;
+            rho[q] = rfn ? rfn(t, nodes[q]) : 1.0;
+        }
+    }
+};
+
+} // namespace llalbm::core
+
+#endif // LLALBM_HIP_POLICIES_HPP
