@@ -360,24 +360,39 @@ int lbm_set_geometry(lbm_ctx *c, const uint8_t *cls)
             if (k == LBM_INLET || k == LBM_OUTLET) edge[(size_t)edge_slot_host(i, j, nr, nc)] = k == LBM_INLET ? kEdgeInlet : kEdgeOutlet;
         }
     // Compact storage: wet nodes only, column-major.  sb holds the compact
-    // index of the first wet node of every (column, 64-row word).
+    // index of the first wet node of every (column, 64-row word).  Each
+    // column starts at the first index congruent to the row of its first
+    // wet node modulo 64 (at most 63 unused slots per column), so a run of
+    // wet rows from there on fills whole, aligned 64-element groups: wave
+    // loads and stores cover full cache lines in the unobstructed parts of
+    // the domain.  Nodes are contiguous inside a column, which is all the
+    // neighbour arithmetic (row i +- 1 = compact index +- 1) relies on.
     std::vector<int32_t> sb((size_t)((nc + 2) * c->wl), 0);
+    int64_t slots = 0;
     {
         int64_t acc = 0;
-        for (int64_t j = 0; j < nc; ++j)
+        for (int64_t j = 0; j < nc; ++j) {
+            const size_t w0 = (size_t)((j + 1) * c->wl + 1);
+            for (int64_t k = 0; k < nrw; ++k)
+                if (wet[w0 + k]) {
+                    const int64_t r0 = 64 * k + __builtin_ctzll(wet[w0 + k]);
+                    acc += ((r0 - acc) % 64 + 64) % 64;
+                    break;
+                }
             for (int64_t k = 0; k < nrw; ++k) {
-                size_t w = (size_t)((j + 1) * c->wl + 1 + k);
-                sb[w] = (int32_t)acc;
-                acc += __builtin_popcountll(wet[w]);
+                sb[w0 + k] = (int32_t)acc;
+                acc += __builtin_popcountll(wet[w0 + k]);
             }
+        }
+        slots = acc;
     }
-    c->plane = std::max<int64_t>(64, ((c->n_wet + 63) / 64) * 64);
+    c->plane = std::max<int64_t>(64, ((slots + 63) / 64) * 64);
     // The kernels address a population as a plane base pointer plus a
     // 32-bit byte offset spanning up to three planes.
     if ((uint64_t)3 * (uint64_t)c->plane * c->esize >= (1ull << 32)) {
         c->nr = c->nc = 0;
         return fail(LBM_ERR_UNSUPPORTED, "lattice with %lld wet nodes exceeds the 32-bit plane addressing "
-                    "(3 * wet nodes * element size must stay below 4 GiB)", (long long)c->n_wet);
+                    "(3 * compact slots * element size must stay below 4 GiB)", (long long)c->n_wet);
     }
     // list order of the reference: row-major scan
     for (int64_t i = 0; i < nr; ++i)

@@ -71,8 +71,38 @@ struct StepArgs {
 // Fills fin_plane / fout_plane from fin / fout.
 void set_planes(StepArgs &a);
 
+// One lattice of a batch (everything step_kernel takes from StepArgs, with
+// both population buffers: the state is in buf[cur ^ parity]).
+struct SampleDesc {
+    Geometry g;
+    const void *buf[2];
+    const void *frozen;
+    const double *bc_prev;
+    const double *bc_cur;
+    int32_t inlet_policy, outlet_policy, collision, cur;
+    int32_t esize, pad_;
+    double bgk_t, bgk_omt, trt_inv_te, trt_a, trt_b;
+    float sh_a, sh_b, scale, inv_scale;
+    int32_t *flag;
+};
+
+struct BatchArgs {
+    const SampleDesc *samples;
+    const int32_t *tiles;  // per tile: sample, column, first row (R segments of 64 rows)
+    int64_t ntiles;
+    int32_t parity;
+};
+
+// Segments of 64 rows per wave the step kernels use by default.
+int32_t default_segments(int32_t storage);
+
 // Kernel launchers (lbm_kernels.hip).  All are asynchronous on `s`.
-hipError_t launch_step(int32_t storage, bool gather, const StepArgs &a, hipStream_t s);
+// R: 64-row segments per wave (1, 2, 4 or 8; <= 0 = default_segments).
+hipError_t launch_step(int32_t storage, bool gather, const StepArgs &a, hipStream_t s, int32_t R = 0);
+// One time step of every lattice of a batch on a persistent grid of
+// `blocks` work-groups (rounded up to a multiple of 8).
+hipError_t launch_batch_step(int32_t storage, bool gather, int32_t R, const BatchArgs &b, int64_t blocks,
+                             hipStream_t s);
 // P (or F* when gather is false) of every node in the reference layout
 // (NR*NC*9 doubles, zeros on non-wet nodes); optional FP64 partial sums
 // per block (5 each) for lbm_reduce.

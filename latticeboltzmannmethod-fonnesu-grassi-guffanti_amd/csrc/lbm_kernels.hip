@@ -42,6 +42,25 @@ __device__ constexpr int kOpp[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
 __device__ constexpr double kW[9] = {4.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0,
                                       1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
 
+// FP32 weights as an unevaluated sum hi + lo (48 significant bits).  The
+// rounded weights alone all carry the same relative error, +7.45e-9 (4/9,
+// 1/9 and 1/36 share one significand), which would give every FP32
+// equilibrium 1 + 7.45e-9 times the node's mass and momentum: a mass
+// source of the same sign at every node and step.
+constexpr float w_hi(double w) { return (float)w; }
+constexpr float w_lo(double w) { return (float)(w - (double)(float)w); }
+__device__ constexpr float kWh[9] = {w_hi(4.0 / 9.0), w_hi(1.0 / 9.0), w_hi(1.0 / 9.0), w_hi(1.0 / 9.0),
+                                     w_hi(1.0 / 9.0), w_hi(1.0 / 36.0), w_hi(1.0 / 36.0), w_hi(1.0 / 36.0),
+                                     w_hi(1.0 / 36.0)};
+__device__ constexpr float kWl[9] = {w_lo(4.0 / 9.0), w_lo(1.0 / 9.0), w_lo(1.0 / 9.0), w_lo(1.0 / 9.0),
+                                     w_lo(1.0 / 9.0), w_lo(1.0 / 36.0), w_lo(1.0 / 36.0), w_lo(1.0 / 36.0),
+                                     w_lo(1.0 / 36.0)};
+// 2/3 and 1/6 of the Zou-He closures in the same form.
+constexpr float k23h = w_hi(2.0 / 3.0), k23l = w_lo(2.0 / 3.0);
+constexpr float k16h = w_hi(1.0 / 6.0), k16l = w_lo(1.0 / 6.0);
+__device__ __forceinline__ float mul23(float x) { return __builtin_fmaf(k23h, x, k23l * x); }
+__device__ __forceinline__ float mul16(float x) { return __builtin_fmaf(k16h, x, k16l * x); }
+
 // Boundary types of the reference's btypes enum (aliases.hpp:28-38).
 enum : int { BT_NONE = 0, BT_TOP, BT_RIGHT, BT_BOTTOM, BT_LEFT, BT_CORNER };
 
@@ -482,27 +501,27 @@ __device__ __forceinline__ void zouhe_velocity32(int bt, float *g, float ux, flo
     if (bt == BT_TOP) {
         dr = ((g0 + g1 + g3 + 2.f * (g2 + g5 + g6)) - uy) / (1.f + uy);
         ru = __builtin_fmaf(dr, ux, ux); rv = __builtin_fmaf(dr, uy, uy);
-        g4 = g2 - (2.f / 3.f) * rv;
-        g7 = g5 - (1.f / 6.f) * rv - 0.5f * ru + 0.5f * (g1 - g3);
-        g8 = g6 - (1.f / 6.f) * rv + 0.5f * ru - 0.5f * (g1 - g3);
+        g4 = g2 - mul23(rv);
+        g7 = g5 - mul16(rv) - 0.5f * ru + 0.5f * (g1 - g3);
+        g8 = g6 - mul16(rv) + 0.5f * ru - 0.5f * (g1 - g3);
     } else if (bt == BT_BOTTOM) {
         dr = ((g0 + g1 + g3 + 2.f * (g4 + g7 + g8)) + uy) / (1.f - uy);
         ru = __builtin_fmaf(dr, ux, ux); rv = __builtin_fmaf(dr, uy, uy);
-        g2 = g4 + (2.f / 3.f) * rv;
-        g5 = g7 + (1.f / 6.f) * rv + 0.5f * ru - 0.5f * (g1 - g3);
-        g6 = g8 + (1.f / 6.f) * rv - 0.5f * ru + 0.5f * (g1 - g3);
+        g2 = g4 + mul23(rv);
+        g5 = g7 + mul16(rv) + 0.5f * ru - 0.5f * (g1 - g3);
+        g6 = g8 + mul16(rv) - 0.5f * ru + 0.5f * (g1 - g3);
     } else if (bt == BT_LEFT) {
         dr = ((g0 + g2 + g4 + 2.f * (g3 + g7 + g6)) + ux) / (1.f - ux);
         ru = __builtin_fmaf(dr, ux, ux); rv = __builtin_fmaf(dr, uy, uy);
-        g1 = g3 + (2.f / 3.f) * ru;
-        g5 = g7 + (1.f / 6.f) * ru + 0.5f * rv - 0.5f * (g2 - g4);
-        g8 = g6 + (1.f / 6.f) * ru - 0.5f * rv + 0.5f * (g2 - g4);
+        g1 = g3 + mul23(ru);
+        g5 = g7 + mul16(ru) + 0.5f * rv - 0.5f * (g2 - g4);
+        g8 = g6 + mul16(ru) - 0.5f * rv + 0.5f * (g2 - g4);
     } else if (bt == BT_RIGHT) {
         dr = ((g0 + g2 + g4 + 2.f * (g1 + g5 + g8)) - ux) / (1.f + ux);
         ru = __builtin_fmaf(dr, ux, ux); rv = __builtin_fmaf(dr, uy, uy);
-        g3 = g1 - (2.f / 3.f) * ru;
-        g7 = g5 - (1.f / 6.f) * ru - 0.5f * rv + 0.5f * (g2 - g4);
-        g6 = g8 - (1.f / 6.f) * ru + 0.5f * rv - 0.5f * (g2 - g4);
+        g3 = g1 - mul23(ru);
+        g7 = g5 - mul16(ru) - 0.5f * rv + 0.5f * (g2 - g4);
+        g6 = g8 - mul16(ru) + 0.5f * rv - 0.5f * (g2 - g4);
     }
     g[1] = g1; g[2] = g2; g[3] = g3; g[4] = g4; g[5] = g5; g[6] = g6; g[7] = g7; g[8] = g8;
 }
@@ -519,21 +538,21 @@ __device__ __forceinline__ void zouhe_pressure32(int bt, float *g, float rho0, f
     else return;
     float ru = rho0 * ux, rv = rho0 * uy;
     if (bt == BT_TOP) {
-        g4 = g2 - (2.f / 3.f) * rv;
-        g7 = g5 - (1.f / 6.f) * rv - 0.5f * ru + 0.5f * (g1 - g3);
-        g8 = g6 - (1.f / 6.f) * rv + 0.5f * ru - 0.5f * (g1 - g3);
+        g4 = g2 - mul23(rv);
+        g7 = g5 - mul16(rv) - 0.5f * ru + 0.5f * (g1 - g3);
+        g8 = g6 - mul16(rv) + 0.5f * ru - 0.5f * (g1 - g3);
     } else if (bt == BT_BOTTOM) {
-        g2 = g4 + (2.f / 3.f) * rv;
-        g5 = g7 + (1.f / 6.f) * rv + 0.5f * ru - 0.5f * (g1 - g3);
-        g6 = g8 + (1.f / 6.f) * rv - 0.5f * ru + 0.5f * (g1 - g3);
+        g2 = g4 + mul23(rv);
+        g5 = g7 + mul16(rv) + 0.5f * ru - 0.5f * (g1 - g3);
+        g6 = g8 + mul16(rv) - 0.5f * ru + 0.5f * (g1 - g3);
     } else if (bt == BT_LEFT) {
-        g1 = g3 + (2.f / 3.f) * ru;
-        g5 = g7 + (1.f / 6.f) * ru + 0.5f * rv - 0.5f * (g2 - g4);
-        g8 = g6 + (1.f / 6.f) * ru - 0.5f * rv + 0.5f * (g2 - g4);
+        g1 = g3 + mul23(ru);
+        g5 = g7 + mul16(ru) + 0.5f * rv - 0.5f * (g2 - g4);
+        g8 = g6 + mul16(ru) - 0.5f * rv + 0.5f * (g2 - g4);
     } else {
-        g3 = g1 - (2.f / 3.f) * ru;
-        g7 = g5 - (1.f / 6.f) * ru - 0.5f * rv + 0.5f * (g2 - g4);
-        g6 = g8 - (1.f / 6.f) * ru + 0.5f * rv - 0.5f * (g2 - g4);
+        g3 = g1 - mul23(ru);
+        g7 = g5 - mul16(ru) - 0.5f * rv + 0.5f * (g2 - g4);
+        g6 = g8 - mul16(ru) + 0.5f * rv - 0.5f * (g2 - g4);
     }
     g[1] = g1; g[2] = g2; g[3] = g3; g[4] = g4; g[5] = g5; g[6] = g6; g[7] = g7; g[8] = g8;
 }
@@ -642,16 +661,23 @@ __device__ __forceinline__ void collide_store_shifted(const StepArgs &a, const f
         a.macro[2 * N + q] = uy;
     }
     // geq_d = feq_d - w_d = w_d (drho + (1 + drho) poly_d),
-    // poly_d = 3 cu + 4.5 cu^2 - 1.5 u^2, as w_d ((poly + drho poly) + drho).
+    // poly_d = 3 cu + 4.5 cu^2 - 1.5 u^2, as w_d ((poly + drho poly) + drho)
+    // with the two-part weights.  The rest equilibrium is the remainder
+    // drho - sum of the moving ones, so the nine carry the node's mass up to
+    // one unbiased rounding.
     const float u15 = 1.5f * (ux * ux + uy * uy);
-    float n_[9];
+    float geq[9];
 #pragma unroll
-    for (int d = 0; d < 9; ++d) {
+    for (int d = 1; d < 9; ++d) {
         float cu = kCxf[d] * ux + kCyf[d] * uy;
         float poly = __builtin_fmaf(cu, __builtin_fmaf(4.5f, cu, 3.f), -u15);
-        float geq = (float)kW[d] * (__builtin_fmaf(drho, poly, poly) + drho);
-        n_[d] = geq - g[d];
+        float x = __builtin_fmaf(drho, poly, poly) + drho;
+        geq[d] = __builtin_fmaf(kWh[d], x, kWl[d] * x);
     }
+    geq[0] = drho - (((geq[1] + geq[3]) + (geq[2] + geq[4])) + ((geq[5] + geq[7]) + (geq[6] + geq[8])));
+    float n_[9];
+#pragma unroll
+    for (int d = 0; d < 9; ++d) n_[d] = geq[d] - g[d];
     // TRT on opposite pairs of n = geq - g:
     //   o_d = g_d + ca (n_d + n_o) + cb (n_d - n_o)
     // (BGK: ca = cb = omega / 2, i.e. o_d = g_d + omega n_d).
@@ -687,23 +713,17 @@ __device__ __forceinline__ void node_general_shifted(const StepArgs &a, int i, i
 template <typename T> struct ComputeOf { using type = float; };
 template <> struct ComputeOf<double> { using type = double; };
 
-// The time-step kernel.  Each wave owns R consecutive 64-row segments of
-// one column.  Waves touching the domain perimeter (and the collide-only
-// first step) run every node through the general path; all other waves
-// first issue the loads of all R segments -- plain streaming for a segment
-// whose 3x3 neighbourhood is all wet, masked bounce-back otherwise -- and
-// only then collide and store, so R*9 loads per lane are in flight.
+// One wave's share of a time step: R consecutive 64-row segments of
+// column j starting at row row0.  Waves touching the domain perimeter (and
+// the collide-only first step) run every node through the general path;
+// all other waves first issue the loads of all R segments -- plain
+// streaming for a segment whose 3x3 neighbourhood is all wet, masked
+// bounce-back otherwise -- and only then collide and store, so R*9 loads
+// per lane are in flight.
 template <typename T, bool GATHER, int R>
-__global__ __launch_bounds__(kBlockRows) void step_kernel(StepArgs a)
+__device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0, int lane)
 {
     using V = typename ComputeOf<T>::type;
-    const int64_t b = xcd_remap(blockIdx.x, a.nblocks);
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    const int64_t gw = b * (kBlockRows / 64) + wave;
-    const int j = (int)(gw / a.wpc);
-    if (j >= a.g.nc) return;
-    const int64_t row0 = (gw - (int64_t)j * a.wpc) * (64 * R);
     if (row0 >= a.g.ldr) return;
     const int nr = a.g.nr;
     const bool perim = (j == 0) | (j == a.g.nc - 1) | (row0 == 0) | (row0 <= nr - 1 && nr - 1 < row0 + 64 * R);
@@ -754,6 +774,78 @@ __global__ __launch_bounds__(kBlockRows) void step_kernel(StepArgs a)
         const int i = (int)row0 + 64 * r + lane;
         if constexpr (sizeof(T) == 8) collide_store_f64(a, p[r], -1, 0, i, j, own[r]);
         else collide_store_shifted<T>(a, p[r], -1, 0, i, j, own[r]);
+    }
+}
+
+// The time-step kernel of one lattice: wave w of the (XCD-remapped) grid
+// owns segments [w mod wpc] * R .. of column w / wpc.
+template <typename T, bool GATHER, int R>
+__global__ __launch_bounds__(kBlockRows) void step_kernel(StepArgs a)
+{
+    const int64_t b = xcd_remap(blockIdx.x, a.nblocks);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int64_t gw = b * (kBlockRows / 64) + wave;
+    const int j = (int)(gw / a.wpc);
+    if (j >= a.g.nc) return;
+    const int64_t row0 = (gw - (int64_t)j * a.wpc) * (64 * R);
+    step_wave<T, GATHER, R>(a, j, row0, lane);
+}
+
+// The time-step kernel of a batch of independent lattices: a persistent
+// grid walks the tile table (sample, column, first row; R segments per
+// tile).  The table is split into 8 contiguous chunks, one per XCD (blocks
+// are dealt round-robin to the XCDs), so each XCD's L2 sees consecutive
+// tiles of the same lattices.  The per-sample description is wave-uniform.
+template <typename T, bool GATHER, int R>
+__global__ __launch_bounds__(kBlockRows) void batch_kernel(BatchArgs bt)
+{
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int64_t xcd = blockIdx.x & 7;
+    const int64_t per_xcd_blocks = (int64_t)gridDim.x >> 3;
+    const int64_t local_wave = (int64_t)(blockIdx.x >> 3) * (kBlockRows / 64) + wave;
+    const int64_t stride = per_xcd_blocks * (kBlockRows / 64);
+    const int64_t chunk = (bt.ntiles + 7) >> 3;
+    const int64_t first = xcd * chunk;
+    const int64_t last = first + chunk < bt.ntiles ? first + chunk : bt.ntiles;
+#pragma unroll 1
+    for (int64_t t = first + local_wave; t < last; t += stride) {
+        const int32_t *tile = bt.tiles + 3 * t;
+        const int s = __builtin_amdgcn_readfirstlane(tile[0]);
+        const int j = __builtin_amdgcn_readfirstlane(tile[1]);
+        const int row0 = __builtin_amdgcn_readfirstlane(tile[2]);
+        const SampleDesc &sd = bt.samples[s];
+        StepArgs a;
+        a.g = sd.g;
+        const int cur = sd.cur ^ bt.parity;
+        a.fin = sd.buf[cur];
+        a.fout = (void *)sd.buf[1 - cur];
+        const size_t stridep = (size_t)sd.g.plane * (size_t)sd.esize;
+#pragma unroll
+        for (int d = 0; d < 9; ++d) {
+            a.fin_plane[d] = (const char *)a.fin + (size_t)d * stridep;
+            a.fout_plane[d] = (char *)a.fout + (size_t)d * stridep;
+        }
+        a.frozen = sd.frozen;
+        a.bc_prev = sd.bc_prev;
+        a.bc_cur = sd.bc_cur;
+        a.inlet_policy = sd.inlet_policy;
+        a.outlet_policy = sd.outlet_policy;
+        a.collision = sd.collision;
+        a.esize = sd.esize;
+        a.bgk_t = sd.bgk_t;
+        a.bgk_omt = sd.bgk_omt;
+        a.trt_inv_te = sd.trt_inv_te;
+        a.trt_a = sd.trt_a;
+        a.trt_b = sd.trt_b;
+        a.sh_a = sd.sh_a;
+        a.sh_b = sd.sh_b;
+        a.scale = sd.scale;
+        a.inv_scale = sd.inv_scale;
+        a.macro = nullptr;
+        a.flag = sd.flag;
+        step_wave<T, GATHER, R>(a, j, row0, lane);
     }
 }
 
@@ -998,6 +1090,37 @@ hipError_t launch_step_t(bool gather, StepArgs a, hipStream_t s)
     return gather ? launch(step_kernel<T, true, R>, nb, s, a) : launch(step_kernel<T, false, R>, nb, s, a);
 }
 
+template <typename T>
+hipError_t launch_step_r(bool gather, int32_t R, const StepArgs &a, hipStream_t s)
+{
+    switch (R) {
+    case 1: return launch_step_t<T, 1>(gather, a, s);
+    case 2: return launch_step_t<T, 2>(gather, a, s);
+    case 8: return launch_step_t<T, 8>(gather, a, s);
+    default: return launch_step_t<T, 4>(gather, a, s);
+    }
+}
+
+template <typename T, int R>
+hipError_t launch_batch_t(bool gather, const BatchArgs &b, int64_t blocks, hipStream_t s)
+{
+    dim3 grid((unsigned)blocks), block(kBlockRows);
+    if (gather) hipLaunchKernelGGL((batch_kernel<T, true, R>), grid, block, 0, s, b);
+    else hipLaunchKernelGGL((batch_kernel<T, false, R>), grid, block, 0, s, b);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_batch_r(bool gather, int32_t R, const BatchArgs &b, int64_t blocks, hipStream_t s)
+{
+    switch (R) {
+    case 1: return launch_batch_t<T, 1>(gather, b, blocks, s);
+    case 2: return launch_batch_t<T, 2>(gather, b, blocks, s);
+    case 8: return launch_batch_t<T, 8>(gather, b, blocks, s);
+    default: return launch_batch_t<T, 4>(gather, b, blocks, s);
+    }
+}
+
 }  // namespace
 
 void set_planes(StepArgs &a)
@@ -1009,12 +1132,27 @@ void set_planes(StepArgs &a)
     }
 }
 
-hipError_t launch_step(int32_t storage, bool gather, const StepArgs &a, hipStream_t s)
+int32_t default_segments(int32_t storage) { return storage == kF64 ? 2 : 4; }
+
+hipError_t launch_step(int32_t storage, bool gather, const StepArgs &a, hipStream_t s, int32_t R)
 {
+    if (R <= 0) R = default_segments(storage);
     switch (storage) {
-    case kF64: return launch_step_t<double, 2>(gather, a, s);
-    case kF32: return launch_step_t<float, 4>(gather, a, s);
-    default: return launch_step_t<_Float16, 4>(gather, a, s);
+    case kF64: return launch_step_r<double>(gather, R, a, s);
+    case kF32: return launch_step_r<float>(gather, R, a, s);
+    default: return launch_step_r<_Float16>(gather, R, a, s);
+    }
+}
+
+hipError_t launch_batch_step(int32_t storage, bool gather, int32_t R, const BatchArgs &b, int64_t blocks,
+                             hipStream_t s)
+{
+    if (R <= 0) R = default_segments(storage);
+    blocks = ((blocks + 7) / 8) * 8;  // whole rounds of the 8 XCDs
+    switch (storage) {
+    case kF64: return launch_batch_r<double>(gather, R, b, blocks, s);
+    case kF32: return launch_batch_r<float>(gather, R, b, blocks, s);
+    default: return launch_batch_r<_Float16>(gather, R, b, blocks, s);
     }
 }
 
