@@ -195,6 +195,58 @@ int lbm_get_stream(lbm_ctx *ctx, void **stream);
 int lbm_get_scale(lbm_ctx *ctx, double *scale);
 
 /* ------------------------------------------------------------------ */
+/* Ensembles of independent lattices                                   */
+/* ------------------------------------------------------------------ */
+/* No counterpart in the reference, which advances one Lattice per
+ * process (Lattice.hpp:287-425).  An ensemble groups lattices that were
+ * set up as usual (lbm_create, lbm_set_geometry, lbm_set_open_values,
+ * lbm_init_equilibrium or lbm_upload_populations) and advances all of them
+ * with ONE persistent kernel launch per time step over a table of tiles
+ * (sample, column, 64*R rows).  Each sample's results are those of
+ * lbm_step on that sample alone, bit for bit.
+ *
+ * Rules: all samples share the device and the storage format and are in
+ * the same state (all freshly initialised/uploaded, or all stepped); a
+ * context belongs to at most one ensemble; while it does, lbm_step and
+ * lbm_set_geometry on it fail with LBM_ERR_STATE and lbm_destroy refuses
+ * it.  Everything else (prescribed values, downloads, reductions, lift and
+ * drag) stays available per sample and is ordered after the ensemble's
+ * steps. */
+typedef struct lbm_batch lbm_batch;
+
+int lbm_batch_create(lbm_ctx *const *samples, int64_t n_samples, lbm_batch **out);
+int lbm_batch_destroy(lbm_batch *batch);
+
+/* Tile shape: `segments` 64-row segments per wave (1, 2, 4 or 8) and a
+ * persistent grid of `blocks` work-groups of 256 threads (rounded up to a
+ * multiple of 8, one share per XCD; 0 = 8 per compute unit). */
+int lbm_batch_set_launch(lbm_batch *batch, int32_t segments, int64_t blocks);
+int lbm_batch_get_launch(lbm_batch *batch, int32_t *segments, int64_t *blocks);
+
+/* Autotuner: times every candidate tile shape and grid size over `steps`
+ * launches each (a launch rewrites the spare buffer from the current
+ * state, so the state does not change) and keeps the fastest.  Reports
+ * the choice and its time per step (any output may be NULL). */
+int lbm_batch_autotune(lbm_batch *batch, int32_t steps, int32_t *segments, int64_t *blocks,
+                       double *ms_per_step);
+
+/* Advances every sample n time steps.  Runs of steps with unchanged
+ * prescribed values replay a captured HIP graph of the step loop unless
+ * LBM_STEP_NO_GRAPH is given; LBM_STEP_CHECK reports LBM_ERR_NAN with the
+ * first sample whose density became NaN.  LBM_STEP_MACRO is not
+ * supported here (step the sample alone for its fields). */
+int lbm_batch_step(lbm_batch *batch, int64_t n, uint32_t flags);
+
+/* Per-sample FP64 reductions of the current pre-collision state, as
+ * lbm_reduce: out[5*s + k] for sample s. */
+int lbm_batch_reduce(lbm_batch *batch, double *out);
+
+/* Total wet nodes over the samples; tiles in the current table. */
+int lbm_batch_info(lbm_batch *batch, int64_t *wet_nodes, int64_t *tiles);
+int lbm_batch_get_stream(lbm_batch *batch, void **stream);
+int lbm_batch_synchronize(lbm_batch *batch);
+
+/* ------------------------------------------------------------------ */
 /* Geometry helpers (host only; write a class plane, ColMajor NR*NC)   */
 /* ------------------------------------------------------------------ */
 

@@ -20,11 +20,12 @@
 #include <vector>
 
 #include "../../include/lbm_amd.h"
+#include "lbm_context.h"
 #include "lbm_internal.h"
 
 using namespace lbm;
 
-namespace {
+namespace lbm {
 
 thread_local std::string g_err;
 
@@ -38,62 +39,6 @@ int fail(int code, const char *fmt, ...)
     g_err = buf;
     return code;
 }
-
-#define HIP_TRY(expr)                                                                                   \
-    do {                                                                                                \
-        hipError_t e_ = (expr);                                                                         \
-        if (e_ != hipSuccess) return fail(LBM_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
-    } while (0)
-
-enum State { kNone = 0, kP = 1, kFstar = 2 };
-
-}  // namespace
-
-struct lbm_ctx {
-    lbm_desc desc{};
-    int32_t storage = kF64;
-    size_t esize = 8;
-    hipStream_t stream = nullptr;
-    // geometry
-    bool has_geom = false;
-    int64_t nr = 0, nc = 0, ldr = 0, plane = 0, wl = 0, E = 0, segs = 0;
-    uint64_t *d_wet = nullptr;
-    int32_t *d_sb = nullptr;
-    uint8_t *d_edge_kind = nullptr;
-    void *d_frozen = nullptr;
-    void *d_F[2] = {nullptr, nullptr};
-    int cur = 0;
-    // prescribed values: two device copies [3][E]; `bc_prev` is the one the
-    // last step used as "current", `bc_cur` the one the next step uses.
-    double *d_bc[2] = {nullptr, nullptr};
-    double *h_bc_pin[2] = {nullptr, nullptr};
-    hipEvent_t bc_ev[2] = {nullptr, nullptr};
-    int bc_cur = 0, bc_prev = 0;
-    std::vector<double> h_bc;  // [3][E] latest values
-    double *d_macro = nullptr;
-    int32_t *d_flag = nullptr;
-    double *d_partials = nullptr;
-    int64_t partials_cap = 0;
-    double *d_out = nullptr;  // 8 doubles
-    int64_t *d_pts = nullptr;
-    int32_t *d_masks = nullptr;
-    double *d_pp = nullptr;
-    int64_t pts_cap = 0;
-    double *d_staging = nullptr;
-    // host-side geometry
-    std::vector<int64_t> open_ij[2], open_e[2];
-    std::vector<int64_t> obst_ij;
-    std::vector<int32_t> obst_mask;
-    int64_t counts[7] = {0, 0, 0, 0, 0, 0, 0};
-    int64_t n_wet = 0;
-    // dynamic state
-    int state = kNone;
-    int64_t t = 0;
-    float scale = 1.f;
-    bool macro_valid = false;
-};
-
-namespace {
 
 StepArgs make_args(const lbm_ctx *c)
 {
@@ -135,6 +80,10 @@ StepArgs make_args(const lbm_ctx *c)
     a.flag = c->d_flag;
     return a;
 }
+
+}  // namespace lbm
+
+namespace {
 
 void free_geometry(lbm_ctx *c)
 {
@@ -283,6 +232,7 @@ int lbm_create(const lbm_desc *desc, lbm_ctx **out)
 int lbm_destroy(lbm_ctx *c)
 {
     if (!c) return LBM_OK;
+    if (c->batch) return fail(LBM_ERR_STATE, "context belongs to an ensemble (destroy the lbm_batch first)");
     (void)hipSetDevice(c->desc.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_geometry(c);
@@ -298,6 +248,7 @@ int lbm_destroy(lbm_ctx *c)
 int lbm_set_relaxation(lbm_ctx *c, double tau, double tau_even, double tau_odd)
 {
     if (!c) return fail(LBM_ERR_INVALID, "null context");
+    c->version++;
     if (c->desc.collision == LBM_BGK && !(tau > 0)) return fail(LBM_ERR_INVALID, "tau must be > 0");
     if (c->desc.collision == LBM_TRT && !(tau_even > 0 && tau_odd > 0))
         return fail(LBM_ERR_INVALID, "tau_even and tau_odd must be > 0");
@@ -310,6 +261,8 @@ int lbm_set_relaxation(lbm_ctx *c, double tau, double tau_even, double tau_odd)
 int lbm_set_geometry(lbm_ctx *c, const uint8_t *cls)
 {
     if (!c || !cls) return fail(LBM_ERR_INVALID, "null argument");
+    if (c->batch) return fail(LBM_ERR_STATE, "context belongs to an ensemble (destroy the lbm_batch first)");
+    c->version++;
     HIP_TRY(hipSetDevice(c->desc.device));
     HIP_TRY(hipStreamSynchronize(c->stream));
     const int64_t nr = c->desc.n_rows, nc = c->desc.n_cols;
@@ -423,6 +376,7 @@ int lbm_set_geometry(lbm_ctx *c, const uint8_t *cls)
     size_t fbytes = (size_t)(9 * c->plane) * c->esize;
     HIP_TRY(hipMalloc(&c->d_wet, wet.size() * sizeof(uint64_t)));
     HIP_TRY(hipMemcpy(c->d_wet, wet.data(), wet.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    c->h_wet = std::move(wet);
     HIP_TRY(hipMalloc(&c->d_sb, sb.size() * sizeof(int32_t)));
     HIP_TRY(hipMemcpy(c->d_sb, sb.data(), sb.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMalloc(&c->d_edge_kind, (size_t)c->E));
@@ -465,6 +419,7 @@ int lbm_open_nodes(lbm_ctx *c, int which, int64_t *ij, int64_t *count)
 
 int lbm_set_open_values(lbm_ctx *c, int which, const double *ux, const double *uy, const double *rho)
 {
+    if (c) c->version++;
     if (!c || (which != 0 && which != 1)) return fail(LBM_ERR_INVALID, "bad argument");
     if (!c->has_geom) return fail(LBM_ERR_STATE, "no geometry");
     HIP_TRY(hipSetDevice(c->desc.device));
@@ -482,6 +437,7 @@ int lbm_set_open_values(lbm_ctx *c, int which, const double *ux, const double *u
 
 int lbm_init_equilibrium(lbm_ctx *c)
 {
+    if (c) c->version++;
     if (!c) return fail(LBM_ERR_INVALID, "null context");
     if (!c->has_geom) return fail(LBM_ERR_STATE, "no geometry");
     HIP_TRY(hipSetDevice(c->desc.device));
@@ -497,6 +453,7 @@ int lbm_init_equilibrium(lbm_ctx *c)
 
 int lbm_upload_populations(lbm_ctx *c, const double *P)
 {
+    if (c) c->version++;
     if (!c || !P) return fail(LBM_ERR_INVALID, "null argument");
     if (!c->has_geom) return fail(LBM_ERR_STATE, "no geometry");
     HIP_TRY(hipSetDevice(c->desc.device));
@@ -547,6 +504,8 @@ int lbm_download_after_collision(lbm_ctx *c, double *P)
 int lbm_step(lbm_ctx *c, int64_t n, uint32_t flags)
 {
     if (!c) return fail(LBM_ERR_INVALID, "null context");
+    if (c->batch) return fail(LBM_ERR_STATE, "context belongs to an ensemble: step it with lbm_batch_step");
+    c->version++;
     if (n < 0) return fail(LBM_ERR_INVALID, "negative step count");
     if (c->state == kNone) return fail(LBM_ERR_STATE, "no populations (call lbm_init_equilibrium)");
     HIP_TRY(hipSetDevice(c->desc.device));

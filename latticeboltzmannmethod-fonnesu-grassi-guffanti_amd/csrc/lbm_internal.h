@@ -71,27 +71,21 @@ struct StepArgs {
 // Fills fin_plane / fout_plane from fin / fout.
 void set_planes(StepArgs &a);
 
-// One lattice of a batch (everything step_kernel takes from StepArgs, with
-// both population buffers: the state is in buf[cur ^ parity]).
-struct SampleDesc {
-    Geometry g;
-    const void *buf[2];
-    const void *frozen;
-    const double *bc_prev;
-    const double *bc_cur;
-    int32_t inlet_policy, outlet_policy, collision, cur;
-    int32_t esize, pad_;
-    double bgk_t, bgk_omt, trt_inv_te, trt_a, trt_b;
-    float sh_a, sh_b, scale, inv_scale;
-    int32_t *flag;
-};
-
+// A batch step: per sample two complete argument sets, args[2 s + p] for
+// buffer parity p (p = 0: input in the sample's current buffer).
 struct BatchArgs {
-    const SampleDesc *samples;
+    const StepArgs *args;
     const int32_t *tiles;  // per tile: sample, column, first row (R segments of 64 rows)
     int64_t ntiles;
     int32_t parity;
 };
+
+// Per-sample FP64 sums (readout's five) of a batch's current state:
+// out[5 s + k].  partials needs 5 * ntiles doubles; tile_begin has
+// nsamples + 1 entries (sample s owns tiles [tile_begin[s], tile_begin[s+1])).
+hipError_t launch_batch_reduce(int32_t storage, bool gather, int32_t R, const BatchArgs &b,
+                               const int64_t *tile_begin, int64_t nsamples, double *partials, double *out,
+                               hipStream_t s);
 
 // Segments of 64 rows per wave the step kernels use by default.
 int32_t default_segments(int32_t storage);

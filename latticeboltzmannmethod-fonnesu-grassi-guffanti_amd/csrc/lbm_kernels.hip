@@ -796,7 +796,10 @@ __global__ __launch_bounds__(kBlockRows) void step_kernel(StepArgs a)
 // grid walks the tile table (sample, column, first row; R segments per
 // tile).  The table is split into 8 contiguous chunks, one per XCD (blocks
 // are dealt round-robin to the XCDs), so each XCD's L2 sees consecutive
-// tiles of the same lattices.  The per-sample description is wave-uniform.
+// tiles of the same lattices.  Each sample's step arguments are prepared on
+// the host (one set per buffer parity) and read in place through a
+// wave-uniform pointer, so they live in scalar registers or are reloaded
+// from the scalar cache instead of occupying vector registers.
 template <typename T, bool GATHER, int R>
 __global__ __launch_bounds__(kBlockRows) void batch_kernel(BatchArgs bt)
 {
@@ -815,37 +818,8 @@ __global__ __launch_bounds__(kBlockRows) void batch_kernel(BatchArgs bt)
         const int s = __builtin_amdgcn_readfirstlane(tile[0]);
         const int j = __builtin_amdgcn_readfirstlane(tile[1]);
         const int row0 = __builtin_amdgcn_readfirstlane(tile[2]);
-        const SampleDesc &sd = bt.samples[s];
-        StepArgs a;
-        a.g = sd.g;
-        const int cur = sd.cur ^ bt.parity;
-        a.fin = sd.buf[cur];
-        a.fout = (void *)sd.buf[1 - cur];
-        const size_t stridep = (size_t)sd.g.plane * (size_t)sd.esize;
-#pragma unroll
-        for (int d = 0; d < 9; ++d) {
-            a.fin_plane[d] = (const char *)a.fin + (size_t)d * stridep;
-            a.fout_plane[d] = (char *)a.fout + (size_t)d * stridep;
-        }
-        a.frozen = sd.frozen;
-        a.bc_prev = sd.bc_prev;
-        a.bc_cur = sd.bc_cur;
-        a.inlet_policy = sd.inlet_policy;
-        a.outlet_policy = sd.outlet_policy;
-        a.collision = sd.collision;
-        a.esize = sd.esize;
-        a.bgk_t = sd.bgk_t;
-        a.bgk_omt = sd.bgk_omt;
-        a.trt_inv_te = sd.trt_inv_te;
-        a.trt_a = sd.trt_a;
-        a.trt_b = sd.trt_b;
-        a.sh_a = sd.sh_a;
-        a.sh_b = sd.sh_b;
-        a.scale = sd.scale;
-        a.inv_scale = sd.inv_scale;
-        a.macro = nullptr;
-        a.flag = sd.flag;
-        step_wave<T, GATHER, R>(a, j, row0, lane);
+        const StepArgs *__restrict__ a = bt.args + 2 * (int64_t)s + bt.parity;
+        step_wave<T, GATHER, R>(*a, j, row0, lane);
     }
 }
 
@@ -880,10 +854,44 @@ __device__ __forceinline__ void block_sum(double *v, double *dst)
     }
 }
 
-// Reads the pre-collision populations P (GATHER, from F*) or the stored
-// state as is (!GATHER) and writes them in FP64 in the reference layout.
-// Optional partial sums: wet mass, fluid momentum x/y, fluid mass, fluid
-// count.
+// Pre-collision populations P of wet node (i, j) in FP64: gathered from F*
+// (GATHER) or read from the stored state as is (!GATHER).  Returns the
+// open-boundary policy of the node (-1 for all other nodes).
+template <typename T, bool GATHER>
+__device__ __forceinline__ int node_state(const StepArgs &a, int i, int j, int lane, const Nbhd &h, uint32_t m,
+                                          double *f)
+{
+    const bool edge = (i == 0) | (i == a.g.nr - 1) | (j == 0) | (j == a.g.nc - 1);
+    const Idx x = lane_idx(h, lane);
+    int64_t e = 0;
+    int policy;
+    if constexpr (sizeof(T) == 8) {
+        policy = load_p64<GATHER>(a, i, j, x, m, edge, e, f);
+    } else {
+        float g[9];
+        policy = load_g<T, GATHER>(a, i, j, x, m, edge, e, g);
+#pragma unroll
+        for (int d = 0; d < 9; ++d) f[d] = (double)g[d] + kW[d];
+    }
+    return policy;
+}
+
+// Adds a node's contribution to the five sums: wet mass, fluid momentum
+// x / y, fluid mass, fluid count.
+__device__ __forceinline__ void add_sums(const double *f, int policy, double *acc)
+{
+    double rho = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
+    acc[0] += rho;
+    if (policy < 0) {
+        acc[1] += f[1] + f[5] + f[8] - (f[3] + f[6] + f[7]);
+        acc[2] += f[2] + f[5] + f[6] - (f[4] + f[7] + f[8]);
+        acc[3] += rho;
+        acc[4] += 1.0;
+    }
+}
+
+// Writes the pre-collision populations P in FP64 in the reference layout
+// (zeros on the other nodes) and/or per-block partial sums (add_sums).
 template <typename T, bool GATHER>
 __global__ __launch_bounds__(kBlockRows) void readout(StepArgs a, double *out, double *partials)
 {
@@ -896,37 +904,69 @@ __global__ __launch_bounds__(kBlockRows) void readout(StepArgs a, double *out, d
         const int64_t N = (int64_t)a.g.nr * a.g.nc;
         const int64_t q = (int64_t)j * a.g.nr + i;
         if (m & 1) {
-            const bool edge = (i == 0) | (i == a.g.nr - 1) | (j == 0) | (j == a.g.nc - 1);
-            const Idx x = lane_idx(h, t.lane);
             double f[9];
-            int64_t e = 0;
-            int policy;
-            if constexpr (sizeof(T) == 8) {
-                policy = load_p64<GATHER>(a, i, j, x, m, edge, e, f);
-            } else {
-                float g[9];
-                policy = load_g<T, GATHER>(a, i, j, x, m, edge, e, g);
-#pragma unroll
-                for (int d = 0; d < 9; ++d) f[d] = (double)g[d] + kW[d];
-            }
+            const int policy = node_state<T, GATHER>(a, i, j, t.lane, h, m, f);
             if (out) {
 #pragma unroll
                 for (int d = 0; d < 9; ++d) out[d * N + q] = f[d];
             }
-            double rho = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
-            acc[0] = rho;
-            if (policy < 0) {
-                acc[1] = f[1] + f[5] + f[8] - (f[3] + f[6] + f[7]);
-                acc[2] = f[2] + f[5] + f[6] - (f[4] + f[7] + f[8]);
-                acc[3] = rho;
-                acc[4] = 1.0;
-            }
+            add_sums(f, policy, acc);
         } else if (out && i < a.g.nr) {
 #pragma unroll
             for (int d = 0; d < 9; ++d) out[d * N + q] = 0.0;
         }
     }
     if (partials) block_sum<5>(acc, partials + (int64_t)blockIdx.x * 5);
+}
+
+// Batch reductions, pass 1: one wave per tile of the batch's tile table
+// sums its nodes (add_sums, FP64) and lane 0 writes partials[5 t + k].
+template <typename T, bool GATHER, int R>
+__global__ __launch_bounds__(kBlockRows) void batch_partials(BatchArgs bt, double *partials)
+{
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+#pragma unroll 1
+    for (int64_t t = (int64_t)blockIdx.x * (kBlockRows / 64) + wave; t < bt.ntiles;
+         t += (int64_t)gridDim.x * (kBlockRows / 64)) {
+        const int32_t *tile = bt.tiles + 3 * t;
+        const int s = __builtin_amdgcn_readfirstlane(tile[0]);
+        const int j = __builtin_amdgcn_readfirstlane(tile[1]);
+        const int row0 = __builtin_amdgcn_readfirstlane(tile[2]);
+        const StepArgs &a = bt.args[2 * (int64_t)s + bt.parity];
+        double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+        for (int r = 0; r < R; ++r) {
+            const int64_t i0 = row0 + 64 * r;
+            if (i0 >= a.g.ldr) break;
+            const Nbhd h = load_nbhd(a.g, j, i0);
+            const uint32_t m = lane_mask(h, lane);
+            if (m & 1) {
+                double f[9];
+                const int policy = node_state<T, GATHER>(a, (int)i0 + lane, j, lane, h, m, f);
+                add_sums(f, policy, acc);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            const double v = wave_sum(acc[k]);
+            if (lane == 0) partials[5 * t + k] = v;
+        }
+    }
+}
+
+// Batch reductions, pass 2: block s folds the partials of sample s's
+// tiles [tile_begin[s], tile_begin[s + 1]) in a fixed order.
+__global__ __launch_bounds__(kBlockRows) void batch_fold(const double *partials, const int64_t *tile_begin,
+                                                         double *out)
+{
+    const int64_t s = blockIdx.x;
+    const int64_t t0 = tile_begin[s], t1 = tile_begin[s + 1];
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += kBlockRows)
+#pragma unroll
+        for (int k = 0; k < 5; ++k) v[k] += partials[5 * t + k];
+    block_sum<5>(v, out + 5 * s);
 }
 
 // Converts a reference-layout FP64 state to the storage format on the wet
@@ -1121,6 +1161,25 @@ hipError_t launch_batch_r(bool gather, int32_t R, const BatchArgs &b, int64_t bl
     }
 }
 
+template <typename T, int R>
+void launch_batch_partials_t(bool gather, const BatchArgs &b, double *partials, hipStream_t s)
+{
+    const int64_t nb = std::min<int64_t>((b.ntiles + 3) / 4, 8192);
+    if (gather) hipLaunchKernelGGL((batch_partials<T, true, R>), dim3((unsigned)nb), dim3(kBlockRows), 0, s, b, partials);
+    else hipLaunchKernelGGL((batch_partials<T, false, R>), dim3((unsigned)nb), dim3(kBlockRows), 0, s, b, partials);
+}
+
+template <typename T>
+void launch_batch_partials_r(bool gather, int32_t R, const BatchArgs &b, double *partials, hipStream_t s)
+{
+    switch (R) {
+    case 1: launch_batch_partials_t<T, 1>(gather, b, partials, s); break;
+    case 2: launch_batch_partials_t<T, 2>(gather, b, partials, s); break;
+    case 8: launch_batch_partials_t<T, 8>(gather, b, partials, s); break;
+    default: launch_batch_partials_t<T, 4>(gather, b, partials, s); break;
+    }
+}
+
 }  // namespace
 
 void set_planes(StepArgs &a)
@@ -1130,6 +1189,24 @@ void set_planes(StepArgs &a)
         a.fin_plane[d] = a.fin ? (const char *)a.fin + (size_t)d * stride : nullptr;
         a.fout_plane[d] = a.fout ? (char *)a.fout + (size_t)d * stride : nullptr;
     }
+}
+
+hipError_t launch_batch_reduce(int32_t storage, bool gather, int32_t R, const BatchArgs &b,
+                               const int64_t *tile_begin, int64_t nsamples, double *partials, double *out,
+                               hipStream_t s)
+{
+    if (R <= 0) R = default_segments(storage);
+    if (b.ntiles > 0) {
+        switch (storage) {
+        case kF64: launch_batch_partials_r<double>(gather, R, b, partials, s); break;
+        case kF32: launch_batch_partials_r<float>(gather, R, b, partials, s); break;
+        default: launch_batch_partials_r<_Float16>(gather, R, b, partials, s); break;
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(batch_fold, dim3((unsigned)nsamples), dim3(kBlockRows), 0, s, partials, tile_begin, out);
+    return hipGetLastError();
 }
 
 int32_t default_segments(int32_t storage) { return storage == kF64 ? 2 : 4; }

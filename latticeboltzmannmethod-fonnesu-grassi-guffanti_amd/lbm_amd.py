@@ -36,7 +36,9 @@ EXPORTED = [
     "lbm_upload_populations", "lbm_download_populations", "lbm_download_after_collision", "lbm_step",
     "lbm_get_time", "lbm_download_macro", "lbm_lift_drag", "lbm_reduce", "lbm_synchronize", "lbm_get_stream",
     "lbm_get_scale", "lbm_geometry_cavity", "lbm_geometry_channel", "lbm_geometry_add_disc",
-    "lbm_geometry_porous",
+    "lbm_geometry_porous", "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_set_launch", "lbm_batch_get_launch",
+    "lbm_batch_autotune", "lbm_batch_step", "lbm_batch_reduce", "lbm_batch_info", "lbm_batch_get_stream",
+    "lbm_batch_synchronize",
 ]
 
 
@@ -100,6 +102,16 @@ def lib():
             "lbm_geometry_add_disc": [C.c_int64, C.c_int64, vp, C.c_int64, C.c_int64, C.c_int64, i64p],
             "lbm_geometry_porous": [C.c_int64, C.c_int64, vp, C.c_uint64, C.c_int64, C.c_int64, C.c_double, dp,
                                     i64p],
+            "lbm_batch_create": [C.POINTER(vp), C.c_int64, C.POINTER(vp)],
+            "lbm_batch_destroy": [vp],
+            "lbm_batch_set_launch": [vp, C.c_int32, C.c_int64],
+            "lbm_batch_get_launch": [vp, i32p, i64p],
+            "lbm_batch_autotune": [vp, C.c_int32, i32p, i64p, dp],
+            "lbm_batch_step": [vp, C.c_int64, C.c_uint32],
+            "lbm_batch_reduce": [vp, dp],
+            "lbm_batch_info": [vp, i64p, i64p],
+            "lbm_batch_get_stream": [vp, C.POINTER(vp)],
+            "lbm_batch_synchronize": [vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -177,7 +189,7 @@ class Lattice:
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().lbm_destroy(self._h)
+            _check(lib().lbm_destroy(self._h))   # refused while the lattice belongs to a Batch
             self._h = None
 
     def __del__(self):
@@ -281,3 +293,70 @@ class Lattice:
         s = C.c_double(0)
         _check(lib().lbm_get_scale(self._h, C.byref(s)))
         return s.value
+
+
+# ---- ensembles -------------------------------------------------------------
+
+class Batch:
+    """An ensemble of Lattice objects advanced together (one lbm_batch).
+
+    The lattices stay usable for downloads, reductions and new prescribed
+    values, but are stepped only through the batch while it exists."""
+
+    def __init__(self, lattices):
+        self.lattices = list(lattices)
+        arr = (C.c_void_p * len(self.lattices))(*[lat._h.value for lat in self.lattices])
+        h = C.c_void_p()
+        _check(lib().lbm_batch_create(arr, len(self.lattices), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lbm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_launch(self, segments, blocks=0):
+        _check(lib().lbm_batch_set_launch(self._h, segments, blocks))
+
+    def launch(self):
+        r, b = C.c_int32(0), C.c_int64(0)
+        _check(lib().lbm_batch_get_launch(self._h, C.byref(r), C.byref(b)))
+        return r.value, b.value
+
+    def autotune(self, steps=5):
+        r, b, ms = C.c_int32(0), C.c_int64(0), C.c_double(0)
+        _check(lib().lbm_batch_autotune(self._h, steps, C.byref(r), C.byref(b), C.byref(ms)))
+        return r.value, b.value, ms.value
+
+    def step(self, n=1, flags=0):
+        _check(lib().lbm_batch_step(self._h, n, flags))
+
+    def reduce(self):
+        out = np.zeros((len(self.lattices), 5))
+        _check(lib().lbm_batch_reduce(self._h, _dp(out)))
+        return out
+
+    def info(self):
+        w, t = C.c_int64(0), C.c_int64(0)
+        _check(lib().lbm_batch_info(self._h, C.byref(w), C.byref(t)))
+        return w.value, t.value
+
+    def stream_handle(self):
+        s = C.c_void_p()
+        _check(lib().lbm_batch_get_stream(self._h, C.byref(s)))
+        return s.value
+
+    def synchronize(self):
+        _check(lib().lbm_batch_synchronize(self._h))

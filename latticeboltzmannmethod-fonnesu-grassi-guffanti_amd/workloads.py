@@ -13,6 +13,8 @@ at full size on the GPU and scaled down where the CPU oracle checks them.
   porous   random discs r in [4, 16] to 30 % solid, BGK tau 0.8, Zou-He
            pressure inlet rho 1.001 / outlet rho 1.000 (8192 x 8192)
   cavity_trt  lid-driven cavity, TRT magic 3/16, u 0.1, Re 1000 (4096 x 4096)
+  ensemble 512 independent porous channels, rows/columns in {128, ..., 1024},
+           porosity in [0.5, 0.9], discs r in [3, 12], the porous set-up
 """
 from __future__ import annotations
 
@@ -133,6 +135,30 @@ def porous(n=8192, seed=3, rmin=4, rmax=16, solid=0.30, tau=0.8, rho_in=1.001, r
                     inlet_policy=L.ZOUHE_PRESSURE, outlet_policy=L.ZOUHE_PRESSURE,
                     inlet=dict(rho=np.full(ni, rho_in)), outlet=dict(rho=np.full(no, rho_out)), steps=20000,
                     info=dict(solid=frac, discs=ndiscs, seed=seed, rho_in=rho_in, rho_out=rho_out))
+
+
+ENSEMBLE_SIZES = tuple(range(128, 1025, 64))
+
+
+def ensemble(n_samples=512, seed=5, sizes=ENSEMBLE_SIZES, porosity=(0.5, 0.9), rmin=3, rmax=12, tau=0.8,
+             rho_in=1.001, rho_out=1.000):
+    """Independent porous channels of uneven sizes: per sample (from one
+    seeded generator) rows and columns drawn uniformly from `sizes`, a
+    porosity uniform in `porosity` (solid fraction 1 - porosity), discs of
+    radius uniform in [rmin, rmax]; the pressure-driven BGK set-up of
+    `porous` (tau 0.8, inlet 1.001 / outlet 1.000)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(n_samples):
+        nc = int(rng.choice(sizes))
+        nr = int(rng.choice(sizes))
+        phi = float(rng.uniform(*porosity))
+        wl = porous(nr, seed=seed * 100003 + s, rmin=rmin, rmax=rmax, solid=1.0 - phi, tau=tau, rho_in=rho_in,
+                    rho_out=rho_out, nc=nc)
+        wl.name = f"sample{s}:{nr}x{nc}"
+        wl.info["porosity_target"] = phi
+        out.append(wl)
+    return out
 
 
 def darcy_permeability(reduced, nu, length, drho, n_all):
