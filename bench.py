@@ -6,7 +6,15 @@ porous medium (discs r in [4, 16], 30 % solid, BGK tau 0.8, Zou-He pressure
 inlet 1.001 / outlet 1.000), populations resident in HBM before the timed
 region.  The headline value is the MLUPS of the FP32 shifted-population
 path; the FP64 (drop-in) and FP16 paths are timed in the same run and
-reported beside it, with the FP32/FP64 velocity error of the sample.
+reported beside it.  The three lattices then run on to the workload's full
+length (20,000 steps) and the FP32/FP16 velocity and density errors
+against the FP64 run, and their FP64-reduced mass difference, are stated.
+
+The ensemble line advances 512 independent porous lattices (rows and
+columns in {128, ..., 1024}, porosity in [0.5, 0.9], discs r in [3, 12])
+with one persistent launch per step, HIP-graph replay and the autotuned
+tile shape, in FP32; its per-sample Darcy permeabilities (FP64 GPU
+reductions) are checked against an FP64 ensemble.
 
 MLUPS = wet nodes * steps / time / 1e6.  Algorithmic HBM bytes per step =
 wet nodes * 18 * sizeof(storage) + NR*NC/8 (bit-packed wet mask); the
@@ -130,13 +138,86 @@ def cpu_baseline(wl, seconds_budget=20.0):
                        f"(reference-structured list sweeps, OpenMP), {threads} threads")
 
 
-def velocity_error(wl, lat_ref, lat, steps):
-    lat_ref.step(1, L.STEP_MACRO)
-    lat.step(1, L.STEP_MACRO)
-    _, u0 = lat_ref.macro()
-    _, u1 = lat.macro()
+def run_to(lat, total_steps):
+    """Steps a lattice until it has done total_steps - 1 steps, then one
+    more step that also writes the rho/u fields."""
+    left = total_steps - lat.time()
+    if left > 1:
+        lat.step(left - 1)
+    lat.step(1, L.STEP_MACRO | L.STEP_CHECK)
+
+
+def stated_errors(wl, lat_ref, lats, total_steps):
+    """L2 relative errors of the fluid-node velocity and density (rho - 1)
+    fields of each shifted-storage lattice against the FP64 one after
+    total_steps steps, and the relative difference of the FP64-reduced
+    wet mass."""
+    run_to(lat_ref, total_steps)
+    r0, u0 = lat_ref.macro()
+    m0 = lat_ref.reduce()[0]
     m = wl.node_class == L.FLUID
-    return float(np.sqrt(np.sum((u1[:, m] - u0[:, m]) ** 2) / max(np.sum(u0[:, m] ** 2), 1e-300)))
+    un = max(np.sum(u0[:, m] ** 2), 1e-300)
+    rn = max(np.sum((r0[m] - 1.0) ** 2), 1e-300)
+    out = {}
+    for s, lat in lats.items():
+        run_to(lat, total_steps)
+        r1, u1 = lat.macro()
+        out[STORAGE_NAMES[s]] = {
+            "velocity_l2_rel": float(np.sqrt(np.sum((u1[:, m] - u0[:, m]) ** 2) / un)),
+            "density_l2_rel": float(np.sqrt(np.sum((r1[m] - r0[m]) ** 2) / rn)),
+            "mass_rel_diff": float((lat.reduce()[0] - m0) / m0),
+        }
+    return {"steps": total_steps, "vs": "f64 GPU path (oracle-pinned)", "by_storage": out}
+
+
+def bench_ensemble(args, ev):
+    """Many independent porous lattices of uneven sizes advanced by one
+    persistent launch per step (lbm_batch), HIP-graph step loop, tile shape
+    from the autotuner."""
+    t0 = time.perf_counter()
+    wls = W.ensemble(n_samples=args.ensemble_samples)
+    wet = sum(w.wet_count() for w in wls)
+    alg = sum(W.algorithmic_bytes(w, L.F32_SHIFTED) for w in wls)
+    lats = [w.make_lattice(L.F32_SHIFTED) for w in wls]
+    log(f"ensemble: {len(wls)} samples, {wet} wet nodes (built in {time.perf_counter() - t0:.1f}s)")
+    b = L.Batch(lats)
+    b.step(args.warmup, L.STEP_CHECK)
+    seg, blocks, tuned_ms = b.autotune(steps=5)
+    b.step(2 * 32, L.STEP_CHECK)                 # capture and warm the graphs
+    b.synchronize()
+    ms = ev.time_ms(b.stream_handle(), lambda: b.step(args.steps))
+    b.step(0, L.STEP_CHECK)
+    per_step_s = ms / 1e3 / args.steps
+    res = dict(samples=len(wls), wet_nodes=wet, storage="f32", mlups=wet / per_step_s / 1e6,
+               ms_per_step=ms / args.steps, alg_bytes_per_step=alg, achieved_gbs=alg / per_step_s / 1e9,
+               frac=alg / per_step_s / 1e9 / HBM_PEAK_GBS, segments=seg, blocks=blocks, graph_steps=32)
+    log(f"  ensemble f32: {res['mlups']:.1f} MLUPS, {res['ms_per_step']:.4f} ms/step, {res['achieved_gbs']:.0f} GB/s "
+        f"= {100 * res['frac']:.1f}% of 8 TB/s (R={seg}, {blocks} work-groups)")
+    if args.ensemble_check_steps > 0:
+        # per-sample Darcy permeability (FP64 GPU reductions) of the FP32
+        # ensemble against the FP64 ensemble (the oracle-pinned path)
+        lats64 = [w.make_lattice(L.F64) for w in wls]
+        b64 = L.Batch(lats64)
+        total = args.ensemble_check_steps
+        b.step(total - lats[0].time(), L.STEP_CHECK)
+        b64.step(total, L.STEP_CHECK)
+        red32, red64 = b.reduce(), b64.reduce()
+        nu = (0.8 - 0.5) / 3.0
+        k32 = np.array([W.darcy_permeability(red32[i], nu, w.nc - 1, 0.001, w.nr * w.nc) for i, w in enumerate(wls)])
+        k64 = np.array([W.darcy_permeability(red64[i], nu, w.nc - 1, 0.001, w.nr * w.nc) for i, w in enumerate(wls)])
+        rel = np.abs(k32 - k64) / np.abs(k64)
+        res["permeability_check"] = dict(steps=total, max_rel_diff_f32_vs_f64=float(rel.max()),
+                                         median_rel_diff=float(np.median(rel)),
+                                         k_range_f64=[float(k64.min()), float(k64.max())])
+        log(f"  permeability after {total} steps: FP32 vs FP64 max rel diff {rel.max():.3e} "
+            f"(median {np.median(rel):.3e}), k in [{k64.min():.3e}, {k64.max():.3e}]")
+        b64.close()
+        for lat in lats64:
+            lat.close()
+    b.close()
+    for lat in lats:
+        lat.close()
+    return res
 
 
 def main():
@@ -148,6 +229,13 @@ def main():
     ap.add_argument("--storage", default="f32", choices=["f64", "f32", "f16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", action="store_true", help="time only the headline storage")
+    ap.add_argument("--error-steps", type=int, default=None,
+                    help="steps after which the shifted-storage errors are stated (default: the workload's run "
+                         "length, 20000 for porous8192; 0 = skip)")
+    ap.add_argument("--ensemble-samples", type=int, default=512)
+    ap.add_argument("--ensemble-check-steps", type=int, default=2000,
+                    help="steps after which the ensemble's FP32 permeabilities are checked against FP64 (0 = skip)")
+    ap.add_argument("--no-ensemble", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -167,16 +255,19 @@ def main():
         results[s] = r
         log(f"  {r['storage']}: {r['mlups']:.1f} MLUPS, {r['ms_per_step']:.4f} ms/step, "
             f"{r['achieved_gbs']:.0f} GB/s = {100 * r['frac']:.1f}% of 8 TB/s")
-        if s in (L.F64, head):
-            lats[s] = lat
-        else:
-            lat.close()
-    err = None
-    if head != L.F64 and L.F64 in lats:
-        err = velocity_error(wl, lats[L.F64], lats[head], args.warmup + args.steps)
-        log(f"  {STORAGE_NAMES[head]} vs f64 velocity L2 rel error after {args.warmup + args.steps + 1} steps: {err:.3e}")
+        lats[s] = lat
+    errors = None
+    error_steps = wl.steps if args.error_steps is None else args.error_steps
+    if error_steps > 0 and L.F64 in lats and len(lats) > 1:
+        t1 = time.perf_counter()
+        errors = stated_errors(wl, lats[L.F64], {s: lat for s, lat in lats.items() if s != L.F64}, error_steps)
+        for name, e in errors["by_storage"].items():
+            log(f"  {name} vs f64 after {error_steps} steps: velocity L2 rel {e['velocity_l2_rel']:.3e}, "
+                f"density L2 rel {e['density_l2_rel']:.3e}, mass rel diff {e['mass_rel_diff']:.3e}")
+        log(f"  (error runs took {time.perf_counter() - t1:.1f}s)")
     for lat in lats.values():
         lat.close()
+    ens = None if args.no_ensemble else bench_ensemble(args, ev)
     cpu = None
     if not args.no_cpu_baseline:
         log("  CPU baseline (oracle) ...")
@@ -204,7 +295,8 @@ def main():
         "cpu_baseline": cpu,
         "storages": {STORAGE_NAMES[s]: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in x.items()}
                      for s, x in results.items()},
-        "error_vs_f64": {"velocity_l2_rel": err, "steps": args.warmup + args.steps + 1} if err is not None else None,
+        "error_vs_f64": errors,
+        "ensemble": ens,
     }
     if L.F64 in results and head != L.F64:
         line["speedup_vs_f64"] = round(r["mlups"] / results[L.F64]["mlups"], 3)

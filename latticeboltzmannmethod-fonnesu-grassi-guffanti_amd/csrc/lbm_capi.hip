@@ -85,8 +85,46 @@ StepArgs make_args(const lbm_ctx *c)
 
 namespace {
 
+// Steps per captured graph: even, so a replay leaves the buffer holding
+// the state unchanged.
+constexpr int64_t kGraphSteps = 32;
+
+void drop_graphs(lbm_ctx *c)
+{
+    for (auto &row : c->graph)
+        for (auto &g : row) {
+            if (g) (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+// Captures kGraphSteps gathering steps from the context's current buffer
+// and prescribed values (both steps of a run after its first).
+int capture_steps(lbm_ctx *c)
+{
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t err = hipSuccess;
+    for (int64_t k = 0; k < kGraphSteps && err == hipSuccess; ++k) {
+        StepArgs a = make_args(c);
+        const int in = c->cur ^ (int)(k & 1);
+        a.fin = c->d_F[in];
+        a.fout = c->d_F[1 - in];
+        a.macro = nullptr;
+        err = launch_step(c->storage, true, a, c->stream);
+    }
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (err != hipSuccess) return fail(LBM_ERR_HIP, "step graph capture: %s", hipGetErrorString(err));
+    if (e2 != hipSuccess) return fail(LBM_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e2));
+    err = hipGraphInstantiate(&c->graph[c->cur][c->bc_cur], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (err != hipSuccess) return fail(LBM_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(err));
+    return LBM_OK;
+}
+
 void free_geometry(lbm_ctx *c)
 {
+    drop_graphs(c);
     (void)hipFree(c->d_wet); c->d_wet = nullptr;
     (void)hipFree(c->d_sb); c->d_sb = nullptr;
     (void)hipFree(c->d_edge_kind); c->d_edge_kind = nullptr;
@@ -255,6 +293,7 @@ int lbm_set_relaxation(lbm_ctx *c, double tau, double tau_even, double tau_odd)
     c->desc.tau = tau;
     c->desc.tau_even = tau_even;
     c->desc.tau_odd = tau_odd;
+    drop_graphs(c);
     return LBM_OK;
 }
 
@@ -442,6 +481,7 @@ int lbm_init_equilibrium(lbm_ctx *c)
     if (!c->has_geom) return fail(LBM_ERR_STATE, "no geometry");
     HIP_TRY(hipSetDevice(c->desc.device));
     c->scale = choose_scale(c);
+    drop_graphs(c);
     StepArgs a = make_args(c);
     HIP_TRY(launch_init_eq(c->storage, a, c->d_F[c->cur], c->d_frozen, c->d_macro, c->stream));
     HIP_TRY(hipMemsetAsync(c->d_flag, 0, 64, c->stream));
@@ -460,6 +500,7 @@ int lbm_upload_populations(lbm_ctx *c, const double *P)
     int r = ensure_staging(c);
     if (r) return r;
     if (c->state == kNone) c->scale = choose_scale(c);
+    drop_graphs(c);
     HIP_TRY(hipMemcpyAsync(c->d_staging, P, (size_t)(9 * c->nr * c->nc) * sizeof(double), hipMemcpyHostToDevice,
                            c->stream));
     StepArgs a = make_args(c);
@@ -509,18 +550,34 @@ int lbm_step(lbm_ctx *c, int64_t n, uint32_t flags)
     if (n < 0) return fail(LBM_ERR_INVALID, "negative step count");
     if (c->state == kNone) return fail(LBM_ERR_STATE, "no populations (call lbm_init_equilibrium)");
     HIP_TRY(hipSetDevice(c->desc.device));
-    for (int64_t s = 0; s < n; ++s) {
+    const bool macro = (flags & LBM_STEP_MACRO) != 0;
+    for (int64_t s = 0; s < n;) {
+        // After the first step of a run (collide-only pass, new prescribed
+        // values) every step is the same launch on alternating buffers:
+        // replay a captured loop of kGraphSteps of them.
+        const bool steady = c->state == kFstar && c->bc_prev == c->bc_cur;
+        if (!(flags & LBM_STEP_NO_GRAPH) && steady && n - s - (macro ? 1 : 0) >= kGraphSteps) {
+            if (!c->graph[c->cur][c->bc_cur]) {
+                int r = capture_steps(c);
+                if (r) return r;
+            }
+            HIP_TRY(hipGraphLaunch(c->graph[c->cur][c->bc_cur], c->stream));
+            c->t += kGraphSteps;
+            s += kGraphSteps;
+            continue;
+        }
         StepArgs a = make_args(c);
         a.fin = c->d_F[c->cur];
         a.fout = c->d_F[1 - c->cur];
-        a.macro = ((flags & LBM_STEP_MACRO) && s == n - 1) ? c->d_macro : nullptr;
+        a.macro = (macro && s == n - 1) ? c->d_macro : nullptr;
         HIP_TRY(launch_step(c->storage, c->state == kFstar, a, c->stream));
         c->cur = 1 - c->cur;
         c->state = kFstar;
         c->bc_prev = c->bc_cur;
         c->t++;
+        ++s;
     }
-    if (n > 0 && (flags & LBM_STEP_MACRO)) c->macro_valid = true;
+    if (n > 0) c->macro_valid = macro;
     if (flags & LBM_STEP_CHECK) {
         int32_t flag = 0;
         HIP_TRY(hipMemcpyAsync(&flag, c->d_flag, sizeof flag, hipMemcpyDeviceToHost, c->stream));

@@ -186,3 +186,30 @@ def test_invalid_geometries_are_rejected():
     with pytest.raises(L.LbmError) as e:
         L.Lattice(bad)
     assert e.value.code == 5
+
+
+def test_graph_replay_equals_eager_steps():
+    """lbm_step replays captured step loops for runs of >= 32 steps; with
+    new prescribed values and downloads between calls, every call must end
+    bit for bit where eager launches end."""
+    wl = W.channel(64, 256, radius=6, centre=(32, 60), re=20)
+    say("case graph replay vs eager, channel 64x256, 5 calls of 70 steps")
+    g = wl.make_lattice(L.F64)
+    e = wl.make_lattice(L.F64)
+    for call in range(5):
+        scale = 1.0 - 0.5 ** (call + 1)
+        for lat in (g, e):
+            lat.set_open_values(0, ux=wl.inlet["ux"] * scale, uy=wl.inlet["uy"])
+        g.step(70)
+        e.step(70, L.STEP_NO_GRAPH)
+        assert np.array_equal(g.populations(), e.populations()), f"call {call}"
+        assert g.lift_drag() == e.lift_drag()
+    orc = oracle_from(wl, threads=THREADS)
+    for call in range(5):
+        scale = 1.0 - 0.5 ** (call + 1)
+        orc.set_open_values(0, ux=wl.inlet["ux"] * scale, uy=wl.inlet["uy"])
+        orc.step(70)
+    compare_state(g, orc, wl.node_class, "graph-replayed channel")
+    for lat in (g, e):
+        lat.close()
+    orc.close()
