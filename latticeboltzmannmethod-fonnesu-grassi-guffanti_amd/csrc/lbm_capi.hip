@@ -111,7 +111,7 @@ int capture_steps(lbm_ctx *c)
         a.fin = c->d_F[in];
         a.fout = c->d_F[1 - in];
         a.macro = nullptr;
-        err = launch_step(c->storage, true, a, c->stream);
+        err = launch_step(c->storage, true, a, c->stream, c->segments);
     }
     hipError_t e2 = hipStreamEndCapture(c->stream, &g);
     if (err != hipSuccess) return fail(LBM_ERR_HIP, "step graph capture: %s", hipGetErrorString(err));
@@ -570,7 +570,7 @@ int lbm_step(lbm_ctx *c, int64_t n, uint32_t flags)
         a.fin = c->d_F[c->cur];
         a.fout = c->d_F[1 - c->cur];
         a.macro = (macro && s == n - 1) ? c->d_macro : nullptr;
-        HIP_TRY(launch_step(c->storage, c->state == kFstar, a, c->stream));
+        HIP_TRY(launch_step(c->storage, c->state == kFstar, a, c->stream, c->segments));
         c->cur = 1 - c->cur;
         c->state = kFstar;
         c->bc_prev = c->bc_cur;
@@ -584,6 +584,64 @@ int lbm_step(lbm_ctx *c, int64_t n, uint32_t flags)
         HIP_TRY(hipStreamSynchronize(c->stream));
         if (flag) return fail(LBM_ERR_NAN, "NaN density at or before step %lld", (long long)c->t);
     }
+    return LBM_OK;
+}
+
+int lbm_set_launch(lbm_ctx *c, int32_t segments)
+{
+    if (!c) return fail(LBM_ERR_INVALID, "null context");
+    if (segments != 0 && segments != 1 && segments != 2 && segments != 4 && segments != 8)
+        return fail(LBM_ERR_INVALID, "segments must be 0, 1, 2, 4 or 8");
+    c->segments = segments;
+    drop_graphs(c);
+    return LBM_OK;
+}
+
+int lbm_get_launch(lbm_ctx *c, int32_t *segments)
+{
+    if (!c || !segments) return fail(LBM_ERR_INVALID, "null argument");
+    *segments = c->segments > 0 ? c->segments : default_segments(c->storage);
+    return LBM_OK;
+}
+
+int lbm_autotune(lbm_ctx *c, int32_t steps, int32_t *segments, double *ms_per_step)
+{
+    if (!c) return fail(LBM_ERR_INVALID, "null context");
+    if (steps < 1) return fail(LBM_ERR_INVALID, "steps must be >= 1");
+    if (c->state == kNone) return fail(LBM_ERR_STATE, "no populations (call lbm_init_equilibrium)");
+    HIP_TRY(hipSetDevice(c->desc.device));
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    HIP_TRY(hipEventCreate(&t0));
+    hipError_t err = hipEventCreate(&t1);
+    float best = -1.f;
+    int32_t best_R = default_segments(c->storage);
+    // Each launch recomputes the spare buffer from the current one, so the
+    // state is the same after tuning.
+    StepArgs a = make_args(c);
+    a.bc_prev = a.bc_cur;
+    a.fin = c->d_F[c->cur];
+    a.fout = c->d_F[1 - c->cur];
+    a.macro = nullptr;
+    for (int32_t R : {1, 2, 4, 8}) {
+        if (err == hipSuccess) err = launch_step(c->storage, true, a, c->stream, R);
+        if (err == hipSuccess) err = hipEventRecord(t0, c->stream);
+        for (int32_t k = 0; k < steps && err == hipSuccess; ++k) err = launch_step(c->storage, true, a, c->stream, R);
+        if (err == hipSuccess) err = hipEventRecord(t1, c->stream);
+        if (err == hipSuccess) err = hipEventSynchronize(t1);
+        float ms = 0.f;
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, t0, t1);
+        if (err == hipSuccess && (best < 0 || ms < best)) {
+            best = ms;
+            best_R = R;
+        }
+    }
+    (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    if (err != hipSuccess) return fail(LBM_ERR_HIP, "autotune: %s", hipGetErrorString(err));
+    c->segments = best_R;
+    drop_graphs(c);
+    if (segments) *segments = best_R;
+    if (ms_per_step) *ms_per_step = best / steps;
     return LBM_OK;
 }
 

@@ -80,6 +80,8 @@ struct lbm_ctx {
     uint64_t version = 0;
     // set while the context belongs to an ensemble (lbm_batch)
     struct lbm_batch *batch = nullptr;
+    // 64-row segments per wave of the step kernel (0 = default_segments)
+    int32_t segments = 0;
     // captured step loops, by [buffer holding the state][prescribed-value copy]
     hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
 };

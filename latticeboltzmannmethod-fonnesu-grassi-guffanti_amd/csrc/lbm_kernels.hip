@@ -190,7 +190,8 @@ __device__ __forceinline__ Idx lane_idx(const Nbhd &h, int lane)
     const int32_t l = h.base[0] + (int32_t)below(h.mid[0]);
     const int32_t c = h.base[1] + (int32_t)below(h.mid[1]);
     const int32_t r = h.base[2] + (int32_t)below(h.mid[2]);
-    const int32_t wl_ = (int32_t)((h.mid[0] >> lane) & 1), wr_ = (int32_t)((h.mid[2] >> lane) & 1);
+    const int32_t wl_ = __builtin_amdgcn_inverse_ballot_w64(h.mid[0]) ? 1 : 0;
+    const int32_t wr_ = __builtin_amdgcn_inverse_ballot_w64(h.mid[2]) ? 1 : 0;
     Idx x;
     x.src[0] = c;           // own node
     x.src[1] = l;           // (i, j-1)
@@ -225,8 +226,8 @@ __device__ __forceinline__ void st(T *__restrict__ base, uint32_t byte_off, T v)
 // node's own opposite slot (bounce-back).  Opposite directions are two
 // planes apart, so both candidates are reached from the lower plane of the
 // pair with a non-negative 32-bit offset.
-template <typename T>
-__device__ __forceinline__ void gather_interior(const StepArgs &a, const Idx &x, uint32_t m, T *p)
+template <typename T, typename WetFn>
+__device__ __forceinline__ void gather_interior_by(const StepArgs &a, const Idx &x, WetFn wet_src, T *p)
 {
     const uint32_t E = (uint32_t)sizeof(T);
     const uint32_t twoP = 2u * (uint32_t)a.g.plane * E;
@@ -234,7 +235,7 @@ __device__ __forceinline__ void gather_interior(const StepArgs &a, const Idx &x,
     p[0] = ld((const T *)a.fin_plane[0], own);
 #pragma unroll
     for (int d = 1; d < 9; ++d) {
-        const bool wet = (m >> d) & 1;
+        const bool wet = wet_src(d);
         const int o = kOpp[d];
         const bool low = d < o;  // d is the lower plane of its pair
         const T *base = (const T *)a.fin_plane[low ? d : o];
@@ -242,6 +243,34 @@ __device__ __forceinline__ void gather_interior(const StepArgs &a, const Idx &x,
         uint32_t ob = own + (low ? twoP : 0u);
         p[d] = ld(base, wet ? nb : ob);
     }
+}
+
+// Per-lane mask form (bit d of m = the source of direction d is wet).
+template <typename T>
+__device__ __forceinline__ void gather_interior(const StepArgs &a, const Idx &x, uint32_t m, T *p)
+{
+    gather_interior_by(a, x, [m](int d) { return ((m >> d) & 1) != 0; }, p);
+}
+
+// Wave-uniform source masks (bit l of src[d] = the source of direction d of
+// lane l is wet): each select is one v_cndmask on a scalar lane mask.
+struct SrcMasks {
+    uint64_t src[9];
+};
+
+__device__ __forceinline__ SrcMasks src_masks(const Nbhd &h)
+{
+    SrcMasks s;
+    s.src[0] = h.mid[1];
+    s.src[1] = h.mid[0];  // (i, j-1)
+    s.src[2] = h.dn[1];   // (i+1, j)
+    s.src[3] = h.mid[2];  // (i, j+1)
+    s.src[4] = h.up[1];   // (i-1, j)
+    s.src[5] = h.dn[0];   // (i+1, j-1)
+    s.src[6] = h.dn[2];   // (i+1, j+1)
+    s.src[7] = h.up[2];   // (i-1, j+1)
+    s.src[8] = h.up[0];   // (i-1, j-1)
+    return s;
 }
 
 // Same for a perimeter node: sources outside the domain read the frozen
@@ -596,99 +625,101 @@ __device__ __forceinline__ int load_g(const StepArgs &a, int i, int j, const Idx
     return policy;
 }
 
-// Compensated running sum (Knuth two-sum per addition, errors folded in
-// at the end): the result is the correctly rounded sum in practice.
-struct CSum {
-    float s = 0.f, c = 0.f;
-    __device__ __forceinline__ void add(float x)
-    {
-        float t = s + x;
-        float bp = t - s;
-        c += (s - (t - bp)) + (x - bp);
-        s = t;
-    }
-    __device__ __forceinline__ float value() const { return s + c; }
+// Moments of shifted populations, summed in FP64.  FP32 values whose
+// exponents lie within 29 binades of each other add up exactly in FP64, so
+// the sums below are exact for shifted populations (and accurate to 2^-53
+// of the largest term otherwise); each result is rounded once.  Opposite
+// pairs are formed first: their sums and differences are reused by the
+// collision.
+struct Moments {
+    float s[4], a[4];  // pair sums / differences, pairs (1,3) (2,4) (5,7) (6,8)
+    double drho, jx, jy;
 };
 
-// Moments of shifted populations: drho = sum g, j = sum c g, compensated.
-__device__ __forceinline__ void moments32(const float *g, float &drho, float &jx, float &jy)
+__device__ __forceinline__ void moments_shifted(const float *g, Moments &m)
 {
-    CSum r, x, y;
-#pragma unroll
-    for (int d = 0; d < 9; ++d) r.add(g[d]);
-    x.add(g[1]); x.add(g[5]); x.add(g[8]); x.add(-g[3]); x.add(-g[6]); x.add(-g[7]);
-    y.add(g[2]); y.add(g[5]); y.add(g[6]); y.add(-g[4]); y.add(-g[7]); y.add(-g[8]);
-    drho = r.value();
-    jx = x.value();
-    jy = y.value();
+    const double g0 = g[0], g1 = g[1], g2 = g[2], g3 = g[3], g4 = g[4], g5 = g[5], g6 = g[6], g7 = g[7], g8 = g[8];
+    const double s0 = g1 + g3, a0 = g1 - g3;
+    const double s1 = g2 + g4, a1 = g2 - g4;
+    const double s2 = g5 + g7, a2 = g5 - g7;
+    const double s3 = g6 + g8, a3 = g6 - g8;
+    m.drho = g0 + ((s0 + s1) + (s2 + s3));
+    // x along +column, y along -row (DefaultEquilibrium.hpp:148-149)
+    m.jx = a0 + (a2 - a3);
+    m.jy = a1 + (a2 + a3);
+    m.s[0] = (float)s0; m.s[1] = (float)s1; m.s[2] = (float)s2; m.s[3] = (float)s3;
+    m.a[0] = (float)a0; m.a[1] = (float)a1; m.a[2] = (float)a2; m.a[3] = (float)a3;
 }
 
-// Physical velocity of each direction (x along +column, y along -row),
-// from the reference's moment sums (DefaultEquilibrium.hpp:148-149).
-__device__ constexpr float kCxf[9] = {0.f, 1.f, 0.f, -1.f, 0.f, 1.f, -1.f, -1.f, 1.f};
-__device__ constexpr float kCyf[9] = {0.f, 0.f, 1.f, 0.f, -1.f, 1.f, 1.f, -1.f, -1.f};
-
 // Moments, equilibrium, collision and store of one wet node on shifted
-// populations, FP32 arithmetic.  Every step is written to keep rounding
-// small relative to the stored values: compensated moment sums, an
-// equilibrium in which 1 + drho is never rounded, and a collision applied
-// to the (small) non-equilibrium part geq - g.
+// populations g = f - w, FP32 arithmetic on exactly summed moments.
+//
+// Per opposite pair (d, q = opp d), with c = c_d and j = rho u:
+//   geq_d + geq_q = 2 w (drho + (4.5 (c.j)^2 - 1.5 j.j) / rho)
+//   geq_d - geq_q = 6 w (c.j)
+// (the D2Q9 equilibrium written in j: rho (3 c.u) = 3 c.j), and the rest
+// equilibrium is the remainder drho - sum of the pairs, so the nine carry
+// the node's mass up to one rounding.  The weights are applied in two
+// parts (hi + lo) so the equilibrium momentum is j to FP32 accuracy, not
+// (1 + 7.45e-9) j.  1/rho only scales the second-order term, where the
+// hardware reciprocal's error is far below FP32 resolution of the result.
+// TRT on pairs of n = geq - g:  o_d = g_d + ca (n_d + n_q) + cb (n_d - n_q)
+// (BGK: ca = cb = omega / 2).
 template <typename T>
 __device__ __forceinline__ void collide_store_shifted(const StepArgs &a, const float *g, int policy, int64_t e, int i,
                                                       int j, int64_t n)
 {
+    Moments m;
+    moments_shifted(g, m);
     float drho, jx, jy;
-    moments32(g, drho, jx, jy);
-    float ux, uy;
+    double rho_out, ux_out, uy_out;
     if (policy == kVelocity || policy == kNoop) {
         // rho = 1 and the prescribed velocity (see collide_store_f64)
-        if (drho != drho) atomicOr(a.flag, 1);
+        if (m.drho != m.drho) atomicOr(a.flag, 1);
         drho = 0.f;
-        ux = (float)a.bc_cur[e];
-        uy = (float)a.bc_cur[a.g.E + e];
+        jx = (float)a.bc_cur[e];
+        jy = (float)a.bc_cur[a.g.E + e];
+        rho_out = 1.0;
+        ux_out = jx;
+        uy_out = jy;
     } else {
-        // Correctly rounded reciprocal: an approximate one is biased and
-        // acts as a small systematic drag on the momentum.
-        float rinv = 1.f / (1.f + drho);
-        ux = jx * rinv;
-        uy = jy * rinv;
-        if (drho != drho) atomicOr(a.flag, 1);
+        if (m.drho != m.drho) atomicOr(a.flag, 1);
+        drho = (float)m.drho;
+        jx = (float)m.jx;
+        jy = (float)m.jy;
+        rho_out = 1.0 + m.drho;
+        ux_out = m.jx / rho_out;
+        uy_out = m.jy / rho_out;
     }
     if (a.macro) {
         int64_t q = (int64_t)j * a.g.nr + i, N = (int64_t)a.g.nr * a.g.nc;
-        a.macro[q] = 1.0 + (double)drho;
-        a.macro[N + q] = ux;
-        a.macro[2 * N + q] = uy;
+        a.macro[q] = rho_out;
+        a.macro[N + q] = ux_out;
+        a.macro[2 * N + q] = uy_out;
     }
-    // geq_d = feq_d - w_d = w_d (drho + (1 + drho) poly_d),
-    // poly_d = 3 cu + 4.5 cu^2 - 1.5 u^2, as w_d ((poly + drho poly) + drho)
-    // with the two-part weights.  The rest equilibrium is the remainder
-    // drho - sum of the moving ones, so the nine carry the node's mass up to
-    // one unbiased rounding.
-    const float u15 = 1.5f * (ux * ux + uy * uy);
-    float geq[9];
-#pragma unroll
-    for (int d = 1; d < 9; ++d) {
-        float cu = kCxf[d] * ux + kCyf[d] * uy;
-        float poly = __builtin_fmaf(cu, __builtin_fmaf(4.5f, cu, 3.f), -u15);
-        float x = __builtin_fmaf(drho, poly, poly) + drho;
-        geq[d] = __builtin_fmaf(kWh[d], x, kWl[d] * x);
-    }
-    geq[0] = drho - (((geq[1] + geq[3]) + (geq[2] + geq[4])) + ((geq[5] + geq[7]) + (geq[6] + geq[8])));
-    float n_[9];
-#pragma unroll
-    for (int d = 0; d < 9; ++d) n_[d] = geq[d] - g[d];
-    // TRT on opposite pairs of n = geq - g:
-    //   o_d = g_d + ca (n_d + n_o) + cb (n_d - n_o)
-    // (BGK: ca = cb = omega / 2, i.e. o_d = g_d + omega n_d).
+    const float rinv = __builtin_amdgcn_rcpf(1.f + drho);
+    const float j15 = 1.5f * __builtin_fmaf(jx, jx, jy * jy);
     const float ca = a.sh_a, cb = a.sh_b;
-    float o[9];
-    o[0] = __builtin_fmaf(2.f * ca, n_[0], g[0]);
+    // c.j of the first direction of each pair: 1 (+x), 2 (+y), 5 (+x+y), 6 (-x+y)
+    const float cj[4] = {jx, jy, jx + jy, jy - jx};
+    constexpr int kD[4] = {1, 2, 5, 6};
+    float o[9], seq_sum = 0.f;
 #pragma unroll
-    for (int d = 1; d < 9; ++d) {
-        const int q = kOpp[d];
-        o[d] = g[d] + __builtin_fmaf(ca, n_[d] + n_[q], cb * (n_[d] - n_[q]));
+    for (int k = 0; k < 4; ++k) {
+        const int d = kD[k], q = kOpp[d];
+        const float c = cj[k];
+        const float sw = drho + __builtin_fmaf(4.5f * c, c, -j15) * rinv;
+        const float seq = __builtin_fmaf(2.f * kWh[d], sw, (2.f * kWl[d]) * sw);
+        const float aeq = __builtin_fmaf(6.f * kWh[d], c, (6.f * kWl[d]) * c);
+        seq_sum += seq;
+        const float ds = seq - m.s[k];
+        const float da = aeq - m.a[k];
+        const float t = cb * da;
+        o[d] = g[d] + __builtin_fmaf(ca, ds, t);
+        o[q] = g[q] + __builtin_fmaf(ca, ds, -t);
     }
+    const float geq0 = drho - seq_sum;
+    o[0] = __builtin_fmaf(2.f * ca, geq0 - g[0], g[0]);
     T raw[9];
 #pragma unroll
     for (int d = 0; d < 9; ++d) raw[d] = from_g<T>(o[d], a.scale);
@@ -755,11 +786,16 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
                 own[r] = x.src[0];
                 // A segment whose 3x3 neighbourhood is all wet (a scalar,
                 // wave-uniform test) streams every direction; otherwise the
-                // per-lane mask selects bounce-back sources.
-                const uint32_t m = wave_is_bulk(h) ? 0x1FFu : lane_mask(h, lane);
-                act[r] = m & 1;
-                T raw[9] = {};
-                if (act[r]) gather_interior(a, x, m, raw);
+                // wave-uniform source masks select bounce-back sources.
+                SrcMasks sm = src_masks(h);
+                if (wave_is_bulk(h))
+#pragma unroll
+                    for (int d = 0; d < 9; ++d) sm.src[d] = ~0ull;
+                act[r] = __builtin_amdgcn_inverse_ballot_w64(sm.src[0]);
+                T raw[9];
+                if (act[r])
+                    gather_interior_by(a, x, [&sm](int d) { return __builtin_amdgcn_inverse_ballot_w64(sm.src[d]); },
+                                       raw);
 #pragma unroll
                 for (int d = 0; d < 9; ++d) {
                     if constexpr (sizeof(T) == 8) p[r][d] = raw[d];

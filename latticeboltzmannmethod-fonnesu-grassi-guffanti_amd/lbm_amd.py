@@ -36,7 +36,7 @@ EXPORTED = [
     "lbm_upload_populations", "lbm_download_populations", "lbm_download_after_collision", "lbm_step",
     "lbm_get_time", "lbm_download_macro", "lbm_lift_drag", "lbm_reduce", "lbm_synchronize", "lbm_get_stream",
     "lbm_get_scale", "lbm_geometry_cavity", "lbm_geometry_channel", "lbm_geometry_add_disc",
-    "lbm_geometry_porous", "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_set_launch", "lbm_batch_get_launch",
+    "lbm_geometry_porous", "lbm_set_launch", "lbm_get_launch", "lbm_autotune", "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_set_launch", "lbm_batch_get_launch",
     "lbm_batch_autotune", "lbm_batch_step", "lbm_batch_reduce", "lbm_batch_info", "lbm_batch_get_stream",
     "lbm_batch_synchronize",
 ]
@@ -102,6 +102,9 @@ def lib():
             "lbm_geometry_add_disc": [C.c_int64, C.c_int64, vp, C.c_int64, C.c_int64, C.c_int64, i64p],
             "lbm_geometry_porous": [C.c_int64, C.c_int64, vp, C.c_uint64, C.c_int64, C.c_int64, C.c_double, dp,
                                     i64p],
+            "lbm_set_launch": [vp, C.c_int32],
+            "lbm_get_launch": [vp, i32p],
+            "lbm_autotune": [vp, C.c_int32, i32p, dp],
             "lbm_batch_create": [C.POINTER(vp), C.c_int64, C.POINTER(vp)],
             "lbm_batch_destroy": [vp],
             "lbm_batch_set_launch": [vp, C.c_int32, C.c_int64],
@@ -282,6 +285,19 @@ class Lattice:
 
     def synchronize(self):
         _check(lib().lbm_synchronize(self._h))
+
+    def set_launch(self, segments):
+        _check(lib().lbm_set_launch(self._h, segments))
+
+    def launch(self):
+        r = C.c_int32(0)
+        _check(lib().lbm_get_launch(self._h, C.byref(r)))
+        return r.value
+
+    def autotune(self, steps=5):
+        r, ms = C.c_int32(0), C.c_double(0)
+        _check(lib().lbm_autotune(self._h, steps, C.byref(r), C.byref(ms)))
+        return r.value, ms.value
 
     def stream_handle(self):
         s = C.c_void_p()

@@ -213,3 +213,26 @@ def test_graph_replay_equals_eager_steps():
     for lat in (g, e):
         lat.close()
     orc.close()
+
+
+def test_autotune_keeps_state_and_results():
+    """The tile-shape autotuner must not change the state, and every tile
+    shape must give the same bits."""
+    wl = W.porous(256, seed=5, nc=320)
+    say("case autotune and tile shapes, porous 256x320 FP64/FP32")
+    for storage in (L.F64, L.F32_SHIFTED):
+        ref = wl.make_lattice(storage)
+        ref.step(80)
+        for seg in (1, 2, 4, 8):
+            lat = wl.make_lattice(storage)
+            lat.step(30)
+            before = lat.populations()
+            if seg == 1:
+                chosen, ms = lat.autotune(steps=2)
+                assert chosen in (1, 2, 4, 8) and ms > 0
+            assert np.array_equal(before, lat.populations())
+            lat.set_launch(seg)
+            lat.step(50)
+            assert np.array_equal(lat.populations(), ref.populations()), (storage, seg)
+            lat.close()
+        ref.close()
