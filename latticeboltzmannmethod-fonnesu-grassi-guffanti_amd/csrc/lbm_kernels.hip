@@ -703,7 +703,7 @@ __device__ __forceinline__ void collide_store_shifted(const StepArgs &a, const f
     // c.j of the first direction of each pair: 1 (+x), 2 (+y), 5 (+x+y), 6 (-x+y)
     const float cj[4] = {jx, jy, jx + jy, jy - jx};
     constexpr int kD[4] = {1, 2, 5, 6};
-    float o[9], seq_sum = 0.f;
+    float o[9], seq_sum;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int d = kD[k], q = kOpp[d];
@@ -711,7 +711,7 @@ __device__ __forceinline__ void collide_store_shifted(const StepArgs &a, const f
         const float sw = drho + __builtin_fmaf(4.5f * c, c, -j15) * rinv;
         const float seq = __builtin_fmaf(2.f * kWh[d], sw, (2.f * kWl[d]) * sw);
         const float aeq = __builtin_fmaf(6.f * kWh[d], c, (6.f * kWl[d]) * c);
-        seq_sum += seq;
+        seq_sum = k == 0 ? seq : seq_sum + seq;
         const float ds = seq - m.s[k];
         const float da = aeq - m.a[k];
         const float t = cb * da;
