@@ -143,7 +143,7 @@ def cpu_baseline(wl, seconds_budget=20.0):
 def run_to(lat, total_steps):
     """Steps a lattice until it has done total_steps - 1 steps, then one
     more step that also writes the rho/u fields."""
-    left = total_steps - lat.time()
+    left = total_steps - lat.time
     if left > 1:
         lat.step(left - 1)
     lat.step(1, L.STEP_MACRO | L.STEP_CHECK)
@@ -201,7 +201,7 @@ def bench_ensemble(args, ev):
         lats64 = [w.make_lattice(L.F64) for w in wls]
         b64 = L.Batch(lats64)
         total = args.ensemble_check_steps
-        b.step(total - lats[0].time(), L.STEP_CHECK)
+        b.step(total - lats[0].time, L.STEP_CHECK)
         b64.step(total, L.STEP_CHECK)
         red32, red64 = b.reduce(), b64.reduce()
         nu = (0.8 - 0.5) / 3.0

@@ -55,6 +55,15 @@ __device__ constexpr float kWh[9] = {w_hi(4.0 / 9.0), w_hi(1.0 / 9.0), w_hi(1.0 
 __device__ constexpr float kWl[9] = {w_lo(4.0 / 9.0), w_lo(1.0 / 9.0), w_lo(1.0 / 9.0), w_lo(1.0 / 9.0),
                                      w_lo(1.0 / 9.0), w_lo(1.0 / 36.0), w_lo(1.0 / 36.0), w_lo(1.0 / 36.0),
                                      w_lo(1.0 / 36.0)};
+// 6 w in the same form (6 * kWh is not exact in FP32: 6 * float(1/9)
+// rounds, which would bias the equilibrium momentum by 2e-8); 2 w is
+// exactly 2 kWh + 2 kWl.
+__device__ constexpr float kW6h[9] = {w_hi(24.0 / 9.0), w_hi(6.0 / 9.0), w_hi(6.0 / 9.0), w_hi(6.0 / 9.0),
+                                      w_hi(6.0 / 9.0), w_hi(6.0 / 36.0), w_hi(6.0 / 36.0), w_hi(6.0 / 36.0),
+                                      w_hi(6.0 / 36.0)};
+__device__ constexpr float kW6l[9] = {w_lo(24.0 / 9.0), w_lo(6.0 / 9.0), w_lo(6.0 / 9.0), w_lo(6.0 / 9.0),
+                                      w_lo(6.0 / 9.0), w_lo(6.0 / 36.0), w_lo(6.0 / 36.0), w_lo(6.0 / 36.0),
+                                      w_lo(6.0 / 36.0)};
 // 2/3 and 1/6 of the Zou-He closures in the same form.
 constexpr float k23h = w_hi(2.0 / 3.0), k23l = w_lo(2.0 / 3.0);
 constexpr float k16h = w_hi(1.0 / 6.0), k16l = w_lo(1.0 / 6.0);
@@ -710,7 +719,7 @@ __device__ __forceinline__ void collide_store_shifted(const StepArgs &a, const f
         const float c = cj[k];
         const float sw = drho + __builtin_fmaf(4.5f * c, c, -j15) * rinv;
         const float seq = __builtin_fmaf(2.f * kWh[d], sw, (2.f * kWl[d]) * sw);
-        const float aeq = __builtin_fmaf(6.f * kWh[d], c, (6.f * kWl[d]) * c);
+        const float aeq = __builtin_fmaf(kW6h[d], c, kW6l[d] * c);
         seq_sum = k == 0 ? seq : seq_sum + seq;
         const float ds = seq - m.s[k];
         const float da = aeq - m.a[k];

@@ -48,7 +48,7 @@ def test_ensemble_equals_lattices_alone(storage):
     with L.Batch(lats) as b:
         for k in schedule:
             b.step(k, L.STEP_CHECK)
-        t = [lat.time() for lat in lats]
+        t = [lat.time for lat in lats]
     assert t == [sum(schedule)] * len(lats)
     worst = 0.0
     for w, lat in zip(wls, lats):
