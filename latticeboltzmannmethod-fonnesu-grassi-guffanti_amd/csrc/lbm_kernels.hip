@@ -754,20 +754,22 @@ template <typename T> struct ComputeOf { using type = float; };
 template <> struct ComputeOf<double> { using type = double; };
 
 // One wave's share of a time step: R consecutive 64-row segments of
-// column j starting at row row0.  Waves touching the domain perimeter (and
-// the collide-only first step) run every node through the general path;
-// all other waves first issue the loads of all R segments -- plain
-// streaming for a segment whose 3x3 neighbourhood is all wet, masked
-// bounce-back otherwise -- and only then collide and store, so R*9 loads
-// per lane are in flight.
+// column j starting at row row0.  Only wet nodes on the domain perimeter
+// need the general path (frozen slots, Zou-He, open nodes): a segment in
+// column 0 or NC-1, or one holding a wet node of row 0 or NR-1, runs
+// through it (as does every segment of the collide-only first step).  All
+// other segments -- including those whose perimeter rows are dry walls,
+// which the zero guard bits turn into bounce-back -- first issue the loads
+// of every such segment (plain streaming when the 3x3 neighbourhood is all
+// wet, selected bounce-back otherwise) and only then collide and store, so
+// up to R*9 loads per lane are in flight.
 template <typename T, bool GATHER, int R>
 __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0, int lane)
 {
     using V = typename ComputeOf<T>::type;
     if (row0 >= a.g.ldr) return;
     const int nr = a.g.nr;
-    const bool perim = (j == 0) | (j == a.g.nc - 1) | (row0 == 0) | (row0 <= nr - 1 && nr - 1 < row0 + 64 * R);
-    if (perim || !GATHER) {
+    if (!GATHER || j == 0 || j == a.g.nc - 1) {
 #pragma unroll 1
         for (int r = 0; r < R; ++r) {
             const int64_t i0 = row0 + 64 * r;
@@ -783,6 +785,7 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
     V p[R][9];
     bool act[R];
     int32_t own[R];
+    uint32_t general = 0;  // segments with a wet node on row 0 or row nr-1
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int64_t i0 = row0 + 64 * r;
@@ -790,7 +793,11 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
         own[r] = 0;
         if (i0 < a.g.ldr) {
             const Nbhd h = load_nbhd(a.g, j, i0);
-            if (h.mid[1] != 0) {
+            const bool top = i0 == 0 && (h.mid[1] & 1);
+            const bool bottom = i0 <= nr - 1 && nr - 1 < i0 + 64 && ((h.mid[1] >> (nr - 1 - i0)) & 1);
+            if (top || bottom) {
+                general |= 1u << r;
+            } else if (h.mid[1] != 0) {
                 const Idx x = lane_idx(h, lane);
                 own[r] = x.src[0];
                 // A segment whose 3x3 neighbourhood is all wet (a scalar,
@@ -819,6 +826,17 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
         const int i = (int)row0 + 64 * r + lane;
         if constexpr (sizeof(T) == 8) collide_store_f64(a, p[r], -1, 0, i, j, own[r]);
         else collide_store_shifted<T>(a, p[r], -1, 0, i, j, own[r]);
+    }
+    if (general) {
+#pragma unroll 1
+        for (int r = 0; r < R; ++r) {
+            if (!((general >> r) & 1)) continue;
+            const int64_t i0 = row0 + 64 * r;
+            const Nbhd h = load_nbhd(a.g, j, i0);
+            const int i = (int)i0 + lane;
+            if constexpr (sizeof(T) == 8) node_general_f64<GATHER>(a, i, j, lane, h);
+            else node_general_shifted<T, GATHER>(a, i, j, lane, h);
+        }
     }
 }
 
