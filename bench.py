@@ -194,7 +194,8 @@ def bench_ensemble(args, ev):
                ms_per_step=ms / args.steps, alg_bytes_per_step=alg, achieved_gbs=alg / per_step_s / 1e9,
                frac=alg / per_step_s / 1e9 / HBM_PEAK_GBS, segments=seg, blocks=blocks, graph_steps=32)
     log(f"  ensemble f32: {res['mlups']:.1f} MLUPS, {res['ms_per_step']:.4f} ms/step, {res['achieved_gbs']:.0f} GB/s "
-        f"= {100 * res['frac']:.1f}% of 8 TB/s (R={seg}, {blocks} work-groups)")
+        f"= {100 * res['frac']:.1f}% of 8 TB/s (R={seg}, "
+        f"{'one wave per tile' if blocks == 0 else f'{blocks} persistent work-groups'})")
     if args.ensemble_check_steps > 0:
         # per-sample Darcy permeability (FP64 GPU reductions) of the FP32
         # ensemble against the FP64 ensemble (the oracle-pinned path)

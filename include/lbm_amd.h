@@ -229,13 +229,15 @@ typedef struct lbm_batch lbm_batch;
 int lbm_batch_create(lbm_ctx *const *samples, int64_t n_samples, lbm_batch **out);
 int lbm_batch_destroy(lbm_batch *batch);
 
-/* Tile shape: `segments` 64-row segments per wave (1, 2, 4 or 8) and a
- * persistent grid of `blocks` work-groups of 256 threads (rounded up to a
- * multiple of 8, one share per XCD; 0 = 8 per compute unit). */
+/* Tile shape: `segments` 64-row segments per wave (1, 2, 4 or 8), and
+ * either a persistent grid of `blocks` work-groups of 256 threads that
+ * walks the tile table (rounded up to a multiple of 8, one share per XCD)
+ * or, with blocks = 0 (the default), one wave per tile. */
 int lbm_batch_set_launch(lbm_batch *batch, int32_t segments, int64_t blocks);
 int lbm_batch_get_launch(lbm_batch *batch, int32_t *segments, int64_t *blocks);
 
-/* Autotuner: times every candidate tile shape and grid size over `steps`
+/* Autotuner: times every candidate tile shape and grid (one wave per tile,
+ * or 2, 4, 8, 16 persistent work-groups per compute unit) over `steps`
  * launches each (a launch rewrites the spare buffer from the current
  * state, so the state does not change) and keeps the fastest.  Reports
  * the choice and its time per step (any output may be NULL). */

@@ -201,6 +201,7 @@ BatchArgs batch_args(const lbm_batch *b, const ArgBuffer &ab, int parity)
     a.tiles = b->d_tiles;
     a.ntiles = b->ntiles;
     a.parity = parity;
+    a.direct = 0;
     return a;
 }
 
@@ -265,7 +266,7 @@ int lbm_batch_create(lbm_ctx *const *samples, int64_t n, lbm_batch **out)
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) == hipSuccess && cus > 0)
         b->cus = cus;
     b->R = default_segments(b->storage);
-    b->blocks = round8(8 * (int64_t)b->cus);
+    b->blocks = 0;  // one wave per tile
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreate(&b->t0);
@@ -313,7 +314,7 @@ int lbm_batch_set_launch(lbm_batch *b, int32_t segments, int64_t blocks)
         return fail(LBM_ERR_INVALID, "segments must be 1, 2, 4 or 8");
     if (blocks < 0 || blocks > (1 << 24)) return fail(LBM_ERR_INVALID, "grid size out of range");
     HIP_TRY(hipSetDevice(b->device));
-    const int64_t nb = blocks == 0 ? round8(8 * (int64_t)b->cus) : round8(blocks);
+    const int64_t nb = blocks == 0 ? 0 : round8(blocks);
     if (segments != b->tiles_R) {
         int r = build_tiles(b, segments);
         if (r) return r;
@@ -342,7 +343,7 @@ int lbm_batch_autotune(lbm_batch *b, int32_t steps, int32_t *segments, int64_t *
     int r = refresh(b, changed);
     if (r) return r;
     const int32_t Rs[] = {1, 2, 4, 8};
-    const int64_t per_cu[] = {2, 4, 8, 16};
+    const int64_t per_cu[] = {0, 2, 4, 8, 16};  // 0: one wave per tile
     float best = -1.f;
     int32_t best_R = b->R;
     int64_t best_nb = b->blocks;
@@ -350,7 +351,7 @@ int lbm_batch_autotune(lbm_batch *b, int32_t steps, int32_t *segments, int64_t *
         r = build_tiles(b, R);
         if (r) return r;
         for (int64_t k : per_cu) {
-            const int64_t nb = round8(k * (int64_t)b->cus);
+            const int64_t nb = k == 0 ? 0 : round8(k * (int64_t)b->cus);
             // every launch recomputes the spare buffer from the current
             // state (fixed parity), so the state is unchanged
             const BatchArgs ba = batch_args(b, b->steady, b->phase);

@@ -78,6 +78,7 @@ struct BatchArgs {
     const int32_t *tiles;  // per tile: sample, column, first row (R segments of 64 rows)
     int64_t ntiles;
     int32_t parity;
+    int32_t direct;  // set by launch_batch_step: one wave per tile instead of a persistent grid
 };
 
 // Per-sample FP64 sums (readout's five) of a batch's current state:
@@ -93,8 +94,9 @@ int32_t default_segments(int32_t storage);
 // Kernel launchers (lbm_kernels.hip).  All are asynchronous on `s`.
 // R: 64-row segments per wave (1, 2, 4 or 8; <= 0 = default_segments).
 hipError_t launch_step(int32_t storage, bool gather, const StepArgs &a, hipStream_t s, int32_t R = 0);
-// One time step of every lattice of a batch on a persistent grid of
-// `blocks` work-groups (rounded up to a multiple of 8).
+// One time step of every lattice of a batch: on a persistent grid of
+// `blocks` work-groups (rounded up to a multiple of 8), or with one wave per
+// tile when blocks <= 0.
 hipError_t launch_batch_step(int32_t storage, bool gather, int32_t R, const BatchArgs &b, int64_t blocks,
                              hipStream_t s);
 // P (or F* when gather is false) of every node in the reference layout

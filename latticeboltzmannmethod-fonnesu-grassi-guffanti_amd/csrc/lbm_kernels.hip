@@ -868,6 +868,18 @@ __global__ __launch_bounds__(kBlockRows) void batch_kernel(BatchArgs bt)
 {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
+    if (bt.direct) {
+        // one wave per tile; each XCD takes one contiguous run of the table
+        const int64_t t = xcd_remap(blockIdx.x, gridDim.x) * (kBlockRows / 64) + wave;
+        if (t >= bt.ntiles) return;
+        const int32_t *tile = bt.tiles + 3 * t;
+        const int s = __builtin_amdgcn_readfirstlane(tile[0]);
+        const int j = __builtin_amdgcn_readfirstlane(tile[1]);
+        const int row0 = __builtin_amdgcn_readfirstlane(tile[2]);
+        const StepArgs *__restrict__ a = bt.args + 2 * (int64_t)s + bt.parity;
+        step_wave<T, GATHER, R>(*a, j, row0, lane);
+        return;
+    }
     const int64_t xcd = blockIdx.x & 7;
     const int64_t per_xcd_blocks = (int64_t)gridDim.x >> 3;
     const int64_t local_wave = (int64_t)(blockIdx.x >> 3) * (kBlockRows / 64) + wave;
@@ -1284,11 +1296,14 @@ hipError_t launch_step(int32_t storage, bool gather, const StepArgs &a, hipStrea
     }
 }
 
-hipError_t launch_batch_step(int32_t storage, bool gather, int32_t R, const BatchArgs &b, int64_t blocks,
+hipError_t launch_batch_step(int32_t storage, bool gather, int32_t R, const BatchArgs &b0, int64_t blocks,
                              hipStream_t s)
 {
     if (R <= 0) R = default_segments(storage);
-    blocks = ((blocks + 7) / 8) * 8;  // whole rounds of the 8 XCDs
+    BatchArgs b = b0;
+    b.direct = blocks <= 0;
+    if (b.direct) blocks = std::max<int64_t>(1, (b.ntiles + kBlockRows / 64 - 1) / (kBlockRows / 64));
+    else blocks = ((blocks + 7) / 8) * 8;  // whole rounds of the 8 XCDs
     switch (storage) {
     case kF64: return launch_batch_r<double>(gather, R, b, blocks, s);
     case kF32: return launch_batch_r<float>(gather, R, b, blocks, s);
