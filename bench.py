@@ -46,6 +46,9 @@ import workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 STORAGE_NAMES = {L.F64: "f64", L.F32_SHIFTED: "f32", L.F16_SHIFTED: "f16"}
+# timed variants: name -> (storage, lbm_desc options)
+VARIANTS = {"f32": (L.F32_SHIFTED, 0), "f64": (L.F64, 0), "f16": (L.F16_SHIFTED, 0),
+            "f16sr": (L.F16_SHIFTED, L.OPT_F16_STOCHASTIC_ROUNDING)}
 
 
 def log(msg):
@@ -94,8 +97,9 @@ def build_workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def time_storage(wl, storage, steps, warmup, ev):
-    lat = wl.make_lattice(storage)
+def time_storage(wl, name, steps, warmup, ev):
+    storage, options = VARIANTS[name]
+    lat = wl.make_lattice(storage, options=options)
     lat.step(warmup, L.STEP_CHECK)
     segments, _ = lat.autotune(steps=5)
     lat.step(2 * 32, L.STEP_CHECK)              # capture and warm the step graphs
@@ -111,7 +115,7 @@ def time_storage(wl, storage, steps, warmup, ev):
     mlups = nwet / per_step_s / 1e6
     alg = W.algorithmic_bytes(wl, storage)
     achieved = alg / per_step_s / 1e9
-    return lat, dict(storage=STORAGE_NAMES[storage], mlups=mlups, mlups_all_nodes=wl.nr * wl.nc / per_step_s / 1e6,
+    return lat, dict(storage=name, mlups=mlups, mlups_all_nodes=wl.nr * wl.nc / per_step_s / 1e6,
                      ms_per_step=ms / steps, wall_s=wall, alg_bytes_per_step=alg, achieved_gbs=achieved,
                      frac=achieved / HBM_PEAK_GBS, segments=segments)
 
@@ -164,7 +168,7 @@ def stated_errors(wl, lat_ref, lats, total_steps):
     for s, lat in lats.items():
         run_to(lat, total_steps)
         r1, u1 = lat.macro()
-        out[STORAGE_NAMES[s]] = {
+        out[s] = {
             "velocity_l2_rel": float(np.sqrt(np.sum((u1[:, m] - u0[:, m]) ** 2) / un)),
             "density_l2_rel": float(np.sqrt(np.sum((r1[m] - r0[m]) ** 2) / rn)),
             "mass_rel_diff": float((lat.reduce()[0] - m0) / m0),
@@ -229,7 +233,7 @@ def main():
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="porous8192")
-    ap.add_argument("--storage", default="f32", choices=["f64", "f32", "f16"])
+    ap.add_argument("--storage", default="f32", choices=list(VARIANTS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", action="store_true", help="time only the headline storage")
     ap.add_argument("--error-steps", type=int, default=None,
@@ -250,8 +254,8 @@ def main():
     wl = build_workload(args.workload)
     log(f"workload {wl.name}: {wl.nr}x{wl.nc}, wet {wl.wet_count()}, info {wl.info} "
         f"(built in {time.perf_counter() - t0:.1f}s)")
-    head = {"f64": L.F64, "f32": L.F32_SHIFTED, "f16": L.F16_SHIFTED}[args.storage]
-    order = [head] if args.only else [head] + [s for s in (L.F64, L.F32_SHIFTED, L.F16_SHIFTED) if s != head]
+    head = args.storage
+    order = [head] if args.only else [head] + [s for s in VARIANTS if s != head]
     results, lats = {}, {}
     for s in order:
         lat, r = time_storage(wl, s, args.steps, args.warmup, ev)
@@ -261,9 +265,9 @@ def main():
         lats[s] = lat
     errors = None
     error_steps = wl.steps if args.error_steps is None else args.error_steps
-    if error_steps > 0 and L.F64 in lats and len(lats) > 1:
+    if error_steps > 0 and "f64" in lats and len(lats) > 1:
         t1 = time.perf_counter()
-        errors = stated_errors(wl, lats[L.F64], {s: lat for s, lat in lats.items() if s != L.F64}, error_steps)
+        errors = stated_errors(wl, lats["f64"], {s: lat for s, lat in lats.items() if s != "f64"}, error_steps)
         for name, e in errors["by_storage"].items():
             log(f"  {name} vs f64 after {error_steps} steps: velocity L2 rel {e['velocity_l2_rel']:.3e}, "
                 f"density L2 rel {e['density_l2_rel']:.3e}, mass rel diff {e['mass_rel_diff']:.3e}")
@@ -288,21 +292,21 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {L.F64: "f64", L.F32_SHIFTED: "f32", L.F16_SHIFTED: "f32"}[head],
+        "dtype": {"f64": "f64", "f32": "f32", "f16": "f32", "f16sr": "f32"}[head],
         "data": "synthetic",
         "config": {"workload": wl.name, "nr": wl.nr, "nc": wl.nc, "wet_nodes": wl.wet_count(),
-                   "storage": STORAGE_NAMES[head], "collision": "BGK" if wl.collision == L.BGK else "TRT",
+                   "storage": head, "collision": "BGK" if wl.collision == L.BGK else "TRT",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(r["frac"], 4), "traffic": None},
         "cpu_baseline": cpu,
-        "storages": {STORAGE_NAMES[s]: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in x.items()}
+        "storages": {s: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in x.items()}
                      for s, x in results.items()},
         "error_vs_f64": errors,
         "ensemble": ens,
     }
-    if L.F64 in results and head != L.F64:
-        line["speedup_vs_f64"] = round(r["mlups"] / results[L.F64]["mlups"], 3)
+    if "f64" in results and head != "f64":
+        line["speedup_vs_f64"] = round(r["mlups"] / results["f64"]["mlups"], 3)
     print(json.dumps(line), flush=True)
 
 

@@ -84,8 +84,16 @@ typedef struct lbm_desc {
     int32_t outlet_policy; /* lbm_open_policy of the outlet list (LatticeConfiguration outlet_policy_t) */
     double f16_scale;      /* FP16 storage scale (power of two); 0 = chosen from the boundary values */
     int32_t device;        /* HIP device ordinal */
-    int32_t reserved;
+    int32_t options;       /* lbm_options bits (0 = none) */
 } lbm_desc;
+
+typedef enum lbm_options {
+    /* FP16 storage rounds stochastically instead of to nearest: unbiased
+     * rounding of slowly varying fields, whose round-to-nearest errors would
+     * otherwise repeat coherently step after step.  No reference
+     * counterpart. */
+    LBM_OPT_F16_STOCHASTIC_ROUNDING = 1
+} lbm_options;
 
 /* Library version string. */
 const char *lbm_version(void);

@@ -64,6 +64,7 @@ struct StepArgs {
     double trt_inv_te, trt_a, trt_b;    // 1/tau+, 0.5/tau+, 0.5/tau-
     float sh_a, sh_b;                   // shifted paths: 0.5/tau+, 0.5/tau- (BGK: both 0.5/tau)
     float scale, inv_scale;             // FP16 storage scale
+    int32_t f16_sr;                     // FP16 storage rounds stochastically
     double *macro;          // optional [3][NC][NR]: rho, ux, uy
     int32_t *flag;          // bit 0: NaN density seen
 };

@@ -730,6 +730,29 @@ __device__ __forceinline__ void collide_store_shifted(const StepArgs &a, const f
     const float geq0 = drho - seq_sum;
     o[0] = __builtin_fmaf(2.f * ca, geq0 - g[0], g[0]);
     T raw[9];
+    if constexpr (sizeof(T) == 2) {
+        if (a.f16_sr) {
+            // Stochastic rounding to FP16: add 13 random bits below the
+            // FP16 cut of the FP32 value and truncate (sign-magnitude, so
+            // the same for both signs).  The bits come from one hash per
+            // node of its index and its new rest population, a different
+            // window per direction.
+            uint32_t h = (uint32_t)n * 0x9E3779B9u ^ __float_as_uint(o[0]);
+            h ^= h >> 16;
+            h *= 0x85EBCA6Bu;
+            h ^= h >> 13;
+            h *= 0xC2B2AE35u;
+            h ^= h >> 16;
+#pragma unroll
+            for (int d = 0; d < 9; ++d) {
+                const uint32_t r = __builtin_amdgcn_alignbit(h, h, 3 * d) & 0x1FFFu;
+                const uint32_t b = (__float_as_uint(o[d] * a.scale) + r) & ~0x1FFFu;
+                raw[d] = (T)__uint_as_float(b);
+            }
+            store9(a, n, raw);
+            return;
+        }
+    }
 #pragma unroll
     for (int d = 0; d < 9; ++d) raw[d] = from_g<T>(o[d], a.scale);
     store9(a, n, raw);

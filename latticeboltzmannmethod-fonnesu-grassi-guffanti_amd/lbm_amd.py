@@ -28,6 +28,7 @@ BGK, TRT = 0, 1
 ZOUHE_VELOCITY, ZOUHE_PRESSURE, OPEN_NOOP = 0, 1, 2
 F64, F32_SHIFTED, F16_SHIFTED = 0, 1, 2
 STEP_MACRO, STEP_CHECK, STEP_NO_GRAPH = 1, 2, 4
+OPT_F16_STOCHASTIC_ROUNDING = 1
 ERR_NAMES = {0: "OK", 1: "INVALID", 2: "HIP", 3: "NAN", 4: "STATE", 5: "UNSUPPORTED", 6: "NOMEM"}
 
 EXPORTED = [
@@ -55,7 +56,7 @@ class lbm_desc(C.Structure):
         ("tau", C.c_double), ("tau_even", C.c_double), ("tau_odd", C.c_double),
         ("inlet_policy", C.c_int32), ("outlet_policy", C.c_int32),
         ("f16_scale", C.c_double),
-        ("device", C.c_int32), ("reserved", C.c_int32),
+        ("device", C.c_int32), ("options", C.c_int32),
     ]
 
 
@@ -174,12 +175,13 @@ class Lattice:
     """A device-resident D2Q9 lattice (one lbm_ctx)."""
 
     def __init__(self, node_class, collision=BGK, tau=1.0, tau_even=1.0, tau_odd=1.0,
-                 inlet_policy=ZOUHE_VELOCITY, outlet_policy=OPEN_NOOP, storage=F64, f16_scale=0.0, device=0):
+                 inlet_policy=ZOUHE_VELOCITY, outlet_policy=OPEN_NOOP, storage=F64, f16_scale=0.0, device=0,
+                 options=0):
         cls = np.ascontiguousarray(node_class, dtype=np.uint8)
         self.nc, self.nr = cls.shape
         d = lbm_desc(n_rows=self.nr, n_cols=self.nc, collision=collision, storage=storage, tau=tau,
                      tau_even=tau_even, tau_odd=tau_odd, inlet_policy=inlet_policy, outlet_policy=outlet_policy,
-                     f16_scale=f16_scale, device=device, reserved=0)
+                     f16_scale=f16_scale, device=device, options=options)
         h = C.c_void_p()
         _check(lib().lbm_create(C.byref(d), C.byref(h)))
         self._h = h

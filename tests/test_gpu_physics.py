@@ -81,12 +81,19 @@ def _l2rel(a, b, m):
     return float(np.sqrt(np.sum((a[..., m] - b[..., m]) ** 2) / np.sum(b[..., m] ** 2)))
 
 
-@pytest.mark.parametrize("storage,tol_u", [(L.F32_SHIFTED, 3e-5), (L.F16_SHIFTED, 5e-2)])
-def test_shifted_storage_error_cavity(storage, tol_u):
+# Velocity L2 error bounds after the stated runs; measured values are 5-10x
+# below them (FP32: cavity 1.2e-7, porous 1.6e-7; FP16 rounded to nearest:
+# 5.2e-3 / 2.4e-3).  Stochastic FP16 rounding is held to the same bound.
+SHIFTED = [(L.F32_SHIFTED, 0, 1e-6), (L.F16_SHIFTED, 0, 2.5e-2),
+           (L.F16_SHIFTED, L.OPT_F16_STOCHASTIC_ROUNDING, 2.5e-2)]
+
+
+@pytest.mark.parametrize("storage,options,tol_u", SHIFTED)
+def test_shifted_storage_error_cavity(storage, options, tol_u):
     wl = W.cavity(129)
-    say(f"case {wl.name}: storage {storage} vs FP64, 5000 steps")
+    say(f"case {wl.name}: storage {storage} options {options} vs FP64, 5000 steps")
     ref = wl.make_lattice(L.F64)
-    lo = wl.make_lattice(storage)
+    lo = wl.make_lattice(storage, options=options)
     ref.step(5000, L.STEP_MACRO)
     lo.step(5000, L.STEP_MACRO)
     r0, u0 = ref.macro()
@@ -98,12 +105,12 @@ def test_shifted_storage_error_cavity(storage, tol_u):
     assert eu < tol_u
 
 
-@pytest.mark.parametrize("storage,tol_u", [(L.F32_SHIFTED, 1e-5), (L.F16_SHIFTED, 5e-2)])
-def test_shifted_storage_error_porous(storage, tol_u):
+@pytest.mark.parametrize("storage,options,tol_u", SHIFTED)
+def test_shifted_storage_error_porous(storage, options, tol_u):
     wl = W.porous(512, seed=5)
-    say(f"case {wl.name}: storage {storage} vs FP64, 2000 steps")
+    say(f"case {wl.name}: storage {storage} options {options} vs FP64, 2000 steps")
     ref = wl.make_lattice(L.F64)
-    lo = wl.make_lattice(storage)
+    lo = wl.make_lattice(storage, options=options)
     ref.step(2000, L.STEP_MACRO)
     lo.step(2000, L.STEP_MACRO)
     r0, u0 = ref.macro()
