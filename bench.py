@@ -144,12 +144,15 @@ def cpu_baseline(wl, seconds_budget=20.0):
                        f"(reference-structured list sweeps, OpenMP), {threads} threads")
 
 
-def run_to(lat, total_steps):
+def run_to(lat, total_steps, name="", chunk=10000):
     """Steps a lattice until it has done total_steps - 1 steps, then one
-    more step that also writes the rho/u fields."""
-    left = total_steps - lat.time
-    if left > 1:
-        lat.step(left - 1)
+    more step that also writes the rho/u fields; reports progress every
+    `chunk` steps."""
+    t0 = time.perf_counter()
+    while total_steps - lat.time > 1:
+        n = min(chunk, total_steps - lat.time - 1)
+        lat.step(n, L.STEP_CHECK)
+        log(f"    {name}: step {lat.time} of {total_steps} ({time.perf_counter() - t0:.1f}s)")
     lat.step(1, L.STEP_MACRO | L.STEP_CHECK)
 
 
@@ -158,7 +161,7 @@ def stated_errors(wl, lat_ref, lats, total_steps):
     fields of each shifted-storage lattice against the FP64 one after
     total_steps steps, and the relative difference of the FP64-reduced
     wet mass."""
-    run_to(lat_ref, total_steps)
+    run_to(lat_ref, total_steps, "f64")
     r0, u0 = lat_ref.macro()
     m0 = lat_ref.reduce()[0]
     m = wl.node_class == L.FLUID
@@ -166,7 +169,7 @@ def stated_errors(wl, lat_ref, lats, total_steps):
     rn = max(np.sum((r0[m] - 1.0) ** 2), 1e-300)
     out = {}
     for s, lat in lats.items():
-        run_to(lat, total_steps)
+        run_to(lat, total_steps, s)
         r1, u1 = lat.macro()
         out[s] = {
             "velocity_l2_rel": float(np.sqrt(np.sum((u1[:, m] - u0[:, m]) ** 2) / un)),
