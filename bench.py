@@ -47,8 +47,9 @@ import workloads as W  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 STORAGE_NAMES = {L.F64: "f64", L.F32_SHIFTED: "f32", L.F16_SHIFTED: "f16"}
 # timed variants: name -> (storage, lbm_desc options)
+# f16: stochastic rounding (the default); f16rn: rounded to nearest
 VARIANTS = {"f32": (L.F32_SHIFTED, 0), "f64": (L.F64, 0), "f16": (L.F16_SHIFTED, 0),
-            "f16sr": (L.F16_SHIFTED, L.OPT_F16_STOCHASTIC_ROUNDING)}
+            "f16rn": (L.F16_SHIFTED, L.OPT_F16_ROUND_TO_NEAREST)}
 
 
 def log(msg):
@@ -85,7 +86,9 @@ def build_workload(name):
         return W.porous(8192)
     if name.startswith("open"):
         # the porous set-up without obstacles (pressure-driven channel)
-        return W.porous(int(name[4:]), solid=0.0)
+        wl = W.porous(int(name[4:]), solid=0.0)
+        wl.name = f"open{wl.nr}x{wl.nc}"
+        return wl
     if name.startswith("porous"):
         return W.porous(int(name[6:]))
     if name == "channel":
@@ -95,6 +98,20 @@ def build_workload(name):
     if name.startswith("cavity"):
         return W.cavity(int(name[6:]) if len(name) > 6 else 129)
     raise SystemExit(f"unknown workload {name}")
+
+
+def measured_traffic(wl, name, segments):
+    """HBM bytes per launch of the step kernel for this workload, storage and
+    tile shape, from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised
+    in profiles/ (calibrated against a plain copy kernel); None when that
+    kernel was not profiled."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)
+    e = table.get("step_kernel", {}).get(f"{wl.name}/{name}/R{segments}")
+    return None if e is None else e["read_bytes"] + e["write_bytes"]
 
 
 def time_storage(wl, name, steps, warmup, ev):
@@ -117,7 +134,8 @@ def time_storage(wl, name, steps, warmup, ev):
     achieved = alg / per_step_s / 1e9
     return lat, dict(storage=name, mlups=mlups, mlups_all_nodes=wl.nr * wl.nc / per_step_s / 1e6,
                      ms_per_step=ms / steps, wall_s=wall, alg_bytes_per_step=alg, achieved_gbs=achieved,
-                     frac=achieved / HBM_PEAK_GBS, segments=segments)
+                     frac=achieved / HBM_PEAK_GBS, segments=segments,
+                     traffic_bytes_per_step=measured_traffic(wl, name, segments))
 
 
 def cpu_baseline(wl, seconds_budget=20.0):
@@ -295,13 +313,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"f64": "f64", "f32": "f32", "f16": "f32", "f16sr": "f32"}[head],
+        "dtype": {"f64": "f64", "f32": "f32", "f16": "f32", "f16rn": "f32"}[head],
         "data": "synthetic",
         "config": {"workload": wl.name, "nr": wl.nr, "nc": wl.nc, "wet_nodes": wl.wet_count(),
                    "storage": head, "collision": "BGK" if wl.collision == L.BGK else "TRT",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(r["frac"], 4), "traffic": None},
+                     "frac": round(r["frac"], 4), "traffic": r["traffic_bytes_per_step"]},
         "cpu_baseline": cpu,
         "storages": {s: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in x.items()}
                      for s, x in results.items()},

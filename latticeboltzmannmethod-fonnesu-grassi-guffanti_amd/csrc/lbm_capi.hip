@@ -77,7 +77,7 @@ StepArgs make_args(const lbm_ctx *c)
     }
     a.scale = c->scale;
     a.inv_scale = 1.f / c->scale;
-    a.f16_sr = (c->desc.options & LBM_OPT_F16_STOCHASTIC_ROUNDING) ? 1 : 0;
+    a.f16_sr = (c->desc.options & LBM_OPT_F16_ROUND_TO_NEAREST) ? 0 : 1;
     a.flag = c->d_flag;
     return a;
 }
@@ -234,7 +234,7 @@ int lbm_create(const lbm_desc *desc, lbm_ctx **out)
         return fail(LBM_ERR_INVALID, "unknown storage");
     for (int32_t p : {desc->inlet_policy, desc->outlet_policy})
         if (p < LBM_ZOUHE_VELOCITY || p > LBM_OPEN_NOOP) return fail(LBM_ERR_INVALID, "unknown open policy");
-    if (desc->options & ~(int32_t)LBM_OPT_F16_STOCHASTIC_ROUNDING) return fail(LBM_ERR_INVALID, "unknown option bits");
+    if (desc->options & ~(int32_t)LBM_OPT_F16_ROUND_TO_NEAREST) return fail(LBM_ERR_INVALID, "unknown option bits");
     if (desc->collision == LBM_BGK && !(desc->tau > 0)) return fail(LBM_ERR_INVALID, "tau must be > 0");
     if (desc->collision == LBM_TRT && !(desc->tau_even > 0 && desc->tau_odd > 0))
         return fail(LBM_ERR_INVALID, "tau_even and tau_odd must be > 0");

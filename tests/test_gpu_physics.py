@@ -83,9 +83,11 @@ def _l2rel(a, b, m):
 
 # Velocity L2 error bounds after the stated runs; measured values are 5-10x
 # below them (FP32: cavity 1.2e-7, porous 1.6e-7; FP16 rounded to nearest:
-# 5.2e-3 / 2.4e-3).  Stochastic FP16 rounding is held to the same bound.
-SHIFTED = [(L.F32_SHIFTED, 0, 1e-6), (L.F16_SHIFTED, 0, 2.5e-2),
-           (L.F16_SHIFTED, L.OPT_F16_STOCHASTIC_ROUNDING, 2.5e-2)]
+# 5.2e-3 / 2.4e-3).  FP16 with its default stochastic rounding is held to
+# the same bound here; over long runs it is far more accurate (see the
+# 4096^2 cavity errors stated by bench.py).
+SHIFTED = [(L.F32_SHIFTED, 0, 1e-6), (L.F16_SHIFTED, L.OPT_F16_ROUND_TO_NEAREST, 2.5e-2),
+           (L.F16_SHIFTED, 0, 2.5e-2)]
 
 
 @pytest.mark.parametrize("storage,options,tol_u", SHIFTED)
