@@ -306,9 +306,18 @@ __device__ __forceinline__ void gather_edge(const StepArgs &a, const T *__restri
 
 // Writes the 9 post-collision populations of the node with compact index n.
 template <typename T>
-__device__ __forceinline__ void store9(const StepArgs &a, int64_t n, const T *o)
+__device__ __forceinline__ void store9(const StepArgs &a, int64_t n, const T *o, bool whole_lines = false)
 {
     const uint32_t b = (uint32_t)n * (uint32_t)sizeof(T);
+    if (whole_lines) {
+        // The wave writes whole, aligned cache lines (64 contiguous wet
+        // nodes from a 64-aligned slot): stream them past the caches.  A
+        // partial line written this way would cost HBM a read-modify-write,
+        // so every other store goes through L2, which merges the pieces.
+#pragma unroll
+        for (int d = 0; d < 9; ++d) __builtin_nontemporal_store(o[d], (T *)((char *)a.fout_plane[d] + b));
+        return;
+    }
 #pragma unroll
     for (int d = 0; d < 9; ++d) st((T *)a.fout_plane[d], b, o[d]);
 }
@@ -456,7 +465,7 @@ __device__ __forceinline__ int load_p64(const StepArgs &a, int i, int j, const I
 // (Lattice.hpp:295; the Zou-He density write is lost on a by-value copy,
 // ZouHePolicy.hpp:58,71) and the prescribed u.
 __device__ __forceinline__ void collide_store_f64(const StepArgs &a, const double *p, int policy, int64_t e, int i,
-                                                  int j, int64_t n)
+                                                  int j, int64_t n, bool whole_lines = false)
 {
     double rho, ux, uy;
     if (policy == kVelocity || policy == kNoop) {
@@ -492,7 +501,7 @@ __device__ __forceinline__ void collide_store_f64(const StepArgs &a, const doubl
             o[d] = p[d] - ca * (p[d] + p[q] - eq[d] - eq[q]) - cb * (p[d] - p[q] - eq[d] + eq[q]);
         }
     }
-    store9(a, n, o);
+    store9(a, n, o, whole_lines);
 }
 
 // One node of any kind (perimeter, open boundary, bounce-back).
@@ -676,7 +685,7 @@ __device__ __forceinline__ void moments_shifted(const float *g, Moments &m)
 // (BGK: ca = cb = omega / 2).
 template <typename T>
 __device__ __forceinline__ void collide_store_shifted(const StepArgs &a, const float *g, int policy, int64_t e, int i,
-                                                      int j, int64_t n)
+                                                      int j, int64_t n, bool whole_lines = false)
 {
     Moments m;
     moments_shifted(g, m);
@@ -749,13 +758,13 @@ __device__ __forceinline__ void collide_store_shifted(const StepArgs &a, const f
                 const uint32_t b = (__float_as_uint(o[d] * a.scale) + r) & ~0x1FFFu;
                 raw[d] = (T)__uint_as_float(b);
             }
-            store9(a, n, raw);
+            store9(a, n, raw, whole_lines);
             return;
         }
     }
 #pragma unroll
     for (int d = 0; d < 9; ++d) raw[d] = from_g<T>(o[d], a.scale);
-    store9(a, n, raw);
+    store9(a, n, raw, whole_lines);
 }
 
 template <typename T, bool GATHER>
@@ -809,6 +818,7 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
     bool act[R];
     int32_t own[R];
     uint32_t general = 0;  // segments with a wet node on row 0 or row nr-1
+    uint32_t lines = 0;    // segments whose stores are whole, aligned lines
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int64_t i0 = row0 + 64 * r;
@@ -827,9 +837,11 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
                 // wave-uniform test) streams every direction; otherwise the
                 // wave-uniform source masks select bounce-back sources.
                 SrcMasks sm = src_masks(h);
-                if (wave_is_bulk(h))
+                if (wave_is_bulk(h)) {
 #pragma unroll
                     for (int d = 0; d < 9; ++d) sm.src[d] = ~0ull;
+                    if ((h.base[1] & 63) == 0) lines |= 1u << r;
+                }
                 act[r] = __builtin_amdgcn_inverse_ballot_w64(sm.src[0]);
                 T raw[9];
                 if (act[r])
@@ -847,8 +859,9 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
     for (int r = 0; r < R; ++r) {
         if (!act[r]) continue;
         const int i = (int)row0 + 64 * r + lane;
-        if constexpr (sizeof(T) == 8) collide_store_f64(a, p[r], -1, 0, i, j, own[r]);
-        else collide_store_shifted<T>(a, p[r], -1, 0, i, j, own[r]);
+        const bool whole = (lines >> r) & 1;
+        if constexpr (sizeof(T) == 8) collide_store_f64(a, p[r], -1, 0, i, j, own[r], whole);
+        else collide_store_shifted<T>(a, p[r], -1, 0, i, j, own[r], whole);
     }
     if (general) {
 #pragma unroll 1
