@@ -314,8 +314,12 @@ __device__ __forceinline__ void store9(const StepArgs &a, int64_t n, const T *o,
         // nodes from a 64-aligned slot): stream them past the caches.  A
         // partial line written this way would cost HBM a read-modify-write,
         // so every other store goes through L2, which merges the pieces.
+        // The empty asm statements keep the compiler from merging these
+        // stores with the plain ones below, which would drop the hint.
+        asm volatile("" ::: "memory");
 #pragma unroll
         for (int d = 0; d < 9; ++d) __builtin_nontemporal_store(o[d], (T *)((char *)a.fout_plane[d] + b));
+        asm volatile("" ::: "memory");
         return;
     }
 #pragma unroll
