@@ -306,24 +306,30 @@ __device__ __forceinline__ void gather_edge(const StepArgs &a, const T *__restri
 
 // Writes the 9 post-collision populations of the node with compact index n.
 template <typename T>
-__device__ __forceinline__ void store9(const StepArgs &a, int64_t n, const T *o, bool whole_lines = false)
+__device__ __forceinline__ void store9(const StepArgs &a, int64_t n, const T *o)
 {
     const uint32_t b = (uint32_t)n * (uint32_t)sizeof(T);
-    if (whole_lines) {
-        // The wave writes whole, aligned cache lines (64 contiguous wet
-        // nodes from a 64-aligned slot): stream them past the caches.  A
-        // partial line written this way would cost HBM a read-modify-write,
-        // so every other store goes through L2, which merges the pieces.
-        // The empty asm statements keep the compiler from merging these
-        // stores with the plain ones below, which would drop the hint.
-        asm volatile("" ::: "memory");
-#pragma unroll
-        for (int d = 0; d < 9; ++d) __builtin_nontemporal_store(o[d], (T *)((char *)a.fout_plane[d] + b));
-        asm volatile("" ::: "memory");
-        return;
-    }
 #pragma unroll
     for (int d = 0; d < 9; ++d) st((T *)a.fout_plane[d], b, o[d]);
+}
+
+// Store of one population of plane d.  whole_lines: the wave writes whole,
+// aligned cache lines (64 contiguous wet nodes from a 64-aligned slot), so
+// the data can stream past the caches; a partial line written that way
+// would cost HBM a read-modify-write, so every other store goes through
+// L2, which merges the pieces.  The empty asm statements keep the compiler
+// from merging the two stores, which would drop the hint.
+template <typename T>
+__device__ __forceinline__ void store_plane(const StepArgs &a, int d, int64_t n, T v, bool whole_lines)
+{
+    T *ptr = (T *)((char *)a.fout_plane[d] + (uint32_t)n * (uint32_t)sizeof(T));
+    if (whole_lines) {
+        asm volatile("" ::: "memory");
+        __builtin_nontemporal_store(v, ptr);
+        asm volatile("" ::: "memory");
+    } else {
+        *ptr = v;
+    }
 }
 
 // ---------------------------------------------------------------- FP64 --
@@ -468,8 +474,8 @@ __device__ __forceinline__ int load_p64(const StepArgs &a, int i, int j, const I
 // moments; velocity-prescribed and no-op nodes keep rho = 1
 // (Lattice.hpp:295; the Zou-He density write is lost on a by-value copy,
 // ZouHePolicy.hpp:58,71) and the prescribed u.
-__device__ __forceinline__ void collide_store_f64(const StepArgs &a, const double *p, int policy, int64_t e, int i,
-                                                  int j, int64_t n, bool whole_lines = false)
+__device__ __forceinline__ void collide_f64(const StepArgs &a, const double *p, int policy, int64_t e, int i, int j,
+                                            double *o)
 {
     double rho, ux, uy;
     if (policy == kVelocity || policy == kNoop) {
@@ -488,7 +494,7 @@ __device__ __forceinline__ void collide_store_f64(const StepArgs &a, const doubl
         a.macro[N + q] = ux;
         a.macro[2 * N + q] = uy;
     }
-    double eq[9], o[9];
+    double eq[9];
     feq64(rho, ux, uy, eq);
     if (a.collision == 0) {
         // BGKCollision.hpp:140-153
@@ -505,7 +511,14 @@ __device__ __forceinline__ void collide_store_f64(const StepArgs &a, const doubl
             o[d] = p[d] - ca * (p[d] + p[q] - eq[d] - eq[q]) - cb * (p[d] - p[q] - eq[d] + eq[q]);
         }
     }
-    store9(a, n, o, whole_lines);
+}
+
+__device__ __forceinline__ void collide_store_f64(const StepArgs &a, const double *p, int policy, int64_t e, int i,
+                                                  int j, int64_t n)
+{
+    double o[9];
+    collide_f64(a, p, policy, e, i, j, o);
+    store9(a, n, o);
 }
 
 // One node of any kind (perimeter, open boundary, bounce-back).
@@ -688,8 +701,8 @@ __device__ __forceinline__ void moments_shifted(const float *g, Moments &m)
 // TRT on pairs of n = geq - g:  o_d = g_d + ca (n_d + n_q) + cb (n_d - n_q)
 // (BGK: ca = cb = omega / 2).
 template <typename T>
-__device__ __forceinline__ void collide_store_shifted(const StepArgs &a, const float *g, int policy, int64_t e, int i,
-                                                      int j, int64_t n, bool whole_lines = false)
+__device__ __forceinline__ void collide_shifted(const StepArgs &a, const float *g, int policy, int64_t e, int i, int j,
+                                                int64_t n, T *raw)
 {
     Moments m;
     moments_shifted(g, m);
@@ -742,7 +755,6 @@ __device__ __forceinline__ void collide_store_shifted(const StepArgs &a, const f
     }
     const float geq0 = drho - seq_sum;
     o[0] = __builtin_fmaf(2.f * ca, geq0 - g[0], g[0]);
-    T raw[9];
     if constexpr (sizeof(T) == 2) {
         if (a.f16_sr) {
             // Stochastic rounding to FP16: add 13 random bits below the
@@ -762,13 +774,20 @@ __device__ __forceinline__ void collide_store_shifted(const StepArgs &a, const f
                 const uint32_t b = (__float_as_uint(o[d] * a.scale) + r) & ~0x1FFFu;
                 raw[d] = (T)__uint_as_float(b);
             }
-            store9(a, n, raw, whole_lines);
             return;
         }
     }
 #pragma unroll
     for (int d = 0; d < 9; ++d) raw[d] = from_g<T>(o[d], a.scale);
-    store9(a, n, raw, whole_lines);
+}
+
+template <typename T>
+__device__ __forceinline__ void collide_store_shifted(const StepArgs &a, const float *g, int policy, int64_t e, int i,
+                                                      int j, int64_t n)
+{
+    T raw[9];
+    collide_shifted<T>(a, g, policy, e, i, j, n, raw);
+    store9(a, n, raw);
 }
 
 template <typename T, bool GATHER>
@@ -859,14 +878,24 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
             }
         }
     }
+    // Collide every segment, then store plane by plane across the segments:
+    // the cache line shared by two consecutive segments' ranges then gets
+    // both of its parts back to back instead of nine stores apart (by then
+    // the wave's L2 share may have evicted the first part, which doubles
+    // that line's HBM write).
+    T out[R][9];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         if (!act[r]) continue;
         const int i = (int)row0 + 64 * r + lane;
-        const bool whole = (lines >> r) & 1;
-        if constexpr (sizeof(T) == 8) collide_store_f64(a, p[r], -1, 0, i, j, own[r], whole);
-        else collide_store_shifted<T>(a, p[r], -1, 0, i, j, own[r], whole);
+        if constexpr (sizeof(T) == 8) collide_f64(a, p[r], -1, 0, i, j, out[r]);
+        else collide_shifted<T>(a, p[r], -1, 0, i, j, own[r], out[r]);
     }
+#pragma unroll
+    for (int d = 0; d < 9; ++d)
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (act[r]) store_plane(a, d, own[r], out[r][d], (lines >> r) & 1);
     if (general) {
 #pragma unroll 1
         for (int r = 0; r < R; ++r) {
