@@ -878,24 +878,39 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
             }
         }
     }
-    // Collide every segment, then store plane by plane across the segments:
-    // the cache line shared by two consecutive segments' ranges then gets
-    // both of its parts back to back instead of nine stores apart (by then
-    // the wave's L2 share may have evicted the first part, which doubles
-    // that line's HBM write).
-    T out[R][9];
+    if constexpr (sizeof(T) == 4 && R == 2) {
+        // FP32, two segments: collide both, then store plane by plane
+        // across them.  The cache line shared by the two segments' ranges
+        // then gets both of its parts back to back instead of nine stores
+        // apart, by which time the wave's share of L2 may have evicted the
+        // first part (measured: +6 % HBM writes in segment order).  With
+        // more segments, or 2- and 8-byte storage, the longer live ranges
+        // cost more in register spills than this saves.
+        T out[R][9];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        if (!act[r]) continue;
-        const int i = (int)row0 + 64 * r + lane;
-        if constexpr (sizeof(T) == 8) collide_f64(a, p[r], -1, 0, i, j, out[r]);
-        else collide_shifted<T>(a, p[r], -1, 0, i, j, own[r], out[r]);
+        for (int r = 0; r < R; ++r) {
+            if (!act[r]) continue;
+            const int i = (int)row0 + 64 * r + lane;
+            collide_shifted<T>(a, p[r], -1, 0, i, j, own[r], out[r]);
+        }
+#pragma unroll
+        for (int d = 0; d < 9; ++d)
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (act[r]) store_plane(a, d, own[r], out[r][d], (lines >> r) & 1);
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (!act[r]) continue;
+            const int i = (int)row0 + 64 * r + lane;
+            T out[9];
+            if constexpr (sizeof(T) == 8) collide_f64(a, p[r], -1, 0, i, j, out);
+            else collide_shifted<T>(a, p[r], -1, 0, i, j, own[r], out);
+            const bool whole = (lines >> r) & 1;
+#pragma unroll
+            for (int d = 0; d < 9; ++d) store_plane(a, d, own[r], out[d], whole);
+        }
     }
-#pragma unroll
-    for (int d = 0; d < 9; ++d)
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-            if (act[r]) store_plane(a, d, own[r], out[r][d], (lines >> r) & 1);
     if (general) {
 #pragma unroll 1
         for (int r = 0; r < R; ++r) {
