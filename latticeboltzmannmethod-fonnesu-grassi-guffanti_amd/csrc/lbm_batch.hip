@@ -55,6 +55,9 @@ struct lbm_batch {
     int64_t ntiles = 0;
     int32_t *d_tiles = nullptr;
     int64_t *d_tile_begin = nullptr;
+    // segments holding a wet perimeter node (the general path's share)
+    int32_t *d_ptiles = nullptr;
+    int64_t nptiles = 0;
     // "first" = each sample's arguments as lbm_step would build them for its
     // next step; "steady" = the same with the previous prescribed values set
     // to the current ones (every step after the first of a run)
@@ -159,6 +162,36 @@ int build_tiles(lbm_batch *b, int32_t R)
     return LBM_OK;
 }
 
+// Perimeter table: every 64-row segment holding a wet perimeter node
+// (column 0 or NC-1, row 0 or NR-1) -- the segments the step kernel leaves
+// to the general path; the same test as perimeter_segment on the device.
+int build_perimeter_tiles(lbm_batch *b)
+{
+    std::vector<int32_t> tiles;
+    for (size_t s = 0; s < b->samples.size(); ++s) {
+        const lbm_ctx *c = b->samples[s];
+        const int64_t nrw = c->ldr / 64, last = c->nr - 1;
+        for (int64_t j = 0; j < c->nc; ++j) {
+            const uint64_t *col = c->h_wet.data() + (size_t)((j + 1) * c->wl + 1);
+            for (int64_t k = 0; k < nrw; ++k) {
+                const uint64_t mid = col[k];
+                if (!mid) continue;
+                const bool edge_col = j == 0 || j == c->nc - 1;
+                const bool top = k == 0 && (mid & 1);
+                const bool bottom = k == last / 64 && ((mid >> (last % 64)) & 1);
+                if (!(edge_col || top || bottom)) continue;
+                tiles.push_back((int32_t)s);
+                tiles.push_back((int32_t)j);
+                tiles.push_back((int32_t)(64 * k));
+            }
+        }
+    }
+    b->nptiles = (int64_t)(tiles.size() / 3);
+    HIP_TRY(hipMalloc(&b->d_ptiles, std::max<size_t>(tiles.size(), 3) * sizeof(int32_t)));
+    HIP_TRY(hipMemcpy(b->d_ptiles, tiles.data(), tiles.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    return LBM_OK;
+}
+
 int64_t round8(int64_t v) { return std::max<int64_t>(8, (v + 7) / 8 * 8); }
 
 // Brings the device arguments up to date with the samples: a sample whose
@@ -200,6 +233,8 @@ BatchArgs batch_args(const lbm_batch *b, const ArgBuffer &ab, int parity)
     a.args = ab.dev;
     a.tiles = b->d_tiles;
     a.ntiles = b->ntiles;
+    a.ptiles = b->d_ptiles;
+    a.nptiles = b->nptiles;
     a.parity = parity;
     a.direct = 0;
     return a;
@@ -279,6 +314,7 @@ int lbm_batch_create(lbm_ctx *const *samples, int64_t n, lbm_batch **out)
     int r = alloc_args(b->first, (size_t)(2 * n));
     if (!r) r = alloc_args(b->steady, (size_t)(2 * n));
     if (!r) r = build_tiles(b, b->R);
+    if (!r) r = build_perimeter_tiles(b);
     if (r) return bail(r);
     for (lbm_ctx *c : b->samples) c->batch = b;
     *out = b;
@@ -296,6 +332,7 @@ int lbm_batch_destroy(lbm_batch *b)
     free_args(b->first);
     free_args(b->steady);
     (void)hipFree(b->d_tiles);
+    (void)hipFree(b->d_ptiles);
     (void)hipFree(b->d_tile_begin);
     (void)hipFree(b->d_flags);
     (void)hipFree(b->d_partials);

@@ -78,6 +78,8 @@ struct BatchArgs {
     const StepArgs *args;
     const int32_t *tiles;  // per tile: sample, column, first row (R segments of 64 rows)
     int64_t ntiles;
+    const int32_t *ptiles;  // perimeter table: segments holding a wet perimeter node (64 rows each)
+    int64_t nptiles;
     int32_t parity;
     int32_t direct;  // set by launch_batch_step: one wave per tile instead of a persistent grid
 };

@@ -313,23 +313,20 @@ __device__ __forceinline__ void store9(const StepArgs &a, int64_t n, const T *o)
     for (int d = 0; d < 9; ++d) st((T *)a.fout_plane[d], b, o[d]);
 }
 
-// Store of one population of plane d.  whole_lines: the wave writes whole,
-// aligned cache lines (64 contiguous wet nodes from a 64-aligned slot), so
-// the data can stream past the caches; a partial line written that way
-// would cost HBM a read-modify-write, so every other store goes through
-// L2, which merges the pieces.  The empty asm statements keep the compiler
-// from merging the two stores, which would drop the hint.
+// The 9 stores of one node of a segment whose wave writes whole, aligned
+// cache lines (64 contiguous wet nodes from a 64-aligned slot): stream
+// them past the caches.  A partial line written that way would cost HBM a
+// read-modify-write, so every other store goes through L2, which merges
+// the pieces.  The empty asm statements keep the compiler from merging
+// these stores with plain ones, which would drop the hint.
 template <typename T>
-__device__ __forceinline__ void store_plane(const StepArgs &a, int d, int64_t n, T v, bool whole_lines)
+__device__ __forceinline__ void store9_stream(const StepArgs &a, int64_t n, const T *o)
 {
-    T *ptr = (T *)((char *)a.fout_plane[d] + (uint32_t)n * (uint32_t)sizeof(T));
-    if (whole_lines) {
-        asm volatile("" ::: "memory");
-        __builtin_nontemporal_store(v, ptr);
-        asm volatile("" ::: "memory");
-    } else {
-        *ptr = v;
-    }
+    const uint32_t b = (uint32_t)n * (uint32_t)sizeof(T);
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int d = 0; d < 9; ++d) __builtin_nontemporal_store(o[d], (T *)((char *)a.fout_plane[d] + b));
+    asm volatile("" ::: "memory");
 }
 
 // ---------------------------------------------------------------- FP64 --
@@ -808,39 +805,57 @@ __device__ __forceinline__ void node_general_shifted(const StepArgs &a, int i, i
 template <typename T> struct ComputeOf { using type = float; };
 template <> struct ComputeOf<double> { using type = double; };
 
+// All wet nodes of one 64-row segment through the general path (perimeter
+// nodes: frozen slots, Zou-He, open nodes; every node of the collide-only
+// first step).
+template <typename T, bool GATHER>
+__device__ __forceinline__ void general_segment(const StepArgs &a, int j, int64_t i0, int lane)
+{
+    const Nbhd h = load_nbhd(a.g, j, i0);
+    if (h.mid[1] == 0) return;
+    const int i = (int)i0 + lane;
+    if constexpr (sizeof(T) == 8) node_general_f64<GATHER>(a, i, j, lane, h);
+    else node_general_shifted<T, GATHER>(a, i, j, lane, h);
+}
+
+// True when the segment of column j starting at row i0 holds a wet node on
+// the domain perimeter (column 0 or NC-1, row 0 or NR-1).  Those segments
+// belong to the perimeter kernel; the step kernel skips them.
+__device__ __forceinline__ bool perimeter_segment(const Geometry &g, int j, int64_t i0, uint64_t mid)
+{
+    if (mid == 0) return false;
+    if (j == 0 || j == g.nc - 1) return true;
+    const int64_t last = g.nr - 1;
+    return (i0 == 0 && (mid & 1)) || (i0 <= last && last < i0 + 64 && ((mid >> (last - i0)) & 1));
+}
+
 // One wave's share of a time step: R consecutive 64-row segments of
-// column j starting at row row0.  Only wet nodes on the domain perimeter
-// need the general path (frozen slots, Zou-He, open nodes): a segment in
-// column 0 or NC-1, or one holding a wet node of row 0 or NR-1, runs
-// through it (as does every segment of the collide-only first step).  All
-// other segments -- including those whose perimeter rows are dry walls,
-// which the zero guard bits turn into bounce-back -- first issue the loads
-// of every such segment (plain streaming when the 3x3 neighbourhood is all
-// wet, selected bounce-back otherwise) and only then collide and store, so
-// up to R*9 loads per lane are in flight.
+// column j starting at row row0.  The collide-only first step (!GATHER)
+// runs every node through the general path.  Otherwise segments holding a
+// wet perimeter node are left to the perimeter kernel, and all others --
+// including those whose perimeter rows are dry walls, which the zero guard
+// bits turn into bounce-back -- first issue the loads of every segment
+// (plain streaming when the 3x3 neighbourhood is all wet, selected
+// bounce-back otherwise) and only then collide and store, so up to R*9
+// loads per lane are in flight.  Keeping the general path out of this
+// kernel keeps its register allocation to what the fast path needs.
 template <typename T, bool GATHER, int R>
 __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0, int lane)
 {
     using V = typename ComputeOf<T>::type;
     if (row0 >= a.g.ldr) return;
-    const int nr = a.g.nr;
-    if (!GATHER || j == 0 || j == a.g.nc - 1) {
+    if (!GATHER) {
 #pragma unroll 1
         for (int r = 0; r < R; ++r) {
             const int64_t i0 = row0 + 64 * r;
             if (i0 >= a.g.ldr) break;
-            const Nbhd h = load_nbhd(a.g, j, i0);
-            if (h.mid[1] == 0) continue;
-            const int i = (int)i0 + lane;
-            if constexpr (sizeof(T) == 8) node_general_f64<GATHER>(a, i, j, lane, h);
-            else node_general_shifted<T, GATHER>(a, i, j, lane, h);
+            general_segment<T, GATHER>(a, j, i0, lane);
         }
         return;
     }
     V p[R][9];
     bool act[R];
     int32_t own[R];
-    uint32_t general = 0;  // segments with a wet node on row 0 or row nr-1
     uint32_t lines = 0;    // segments whose stores are whole, aligned lines
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -849,11 +864,7 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
         own[r] = 0;
         if (i0 < a.g.ldr) {
             const Nbhd h = load_nbhd(a.g, j, i0);
-            const bool top = i0 == 0 && (h.mid[1] & 1);
-            const bool bottom = i0 <= nr - 1 && nr - 1 < i0 + 64 && ((h.mid[1] >> (nr - 1 - i0)) & 1);
-            if (top || bottom) {
-                general |= 1u << r;
-            } else if (h.mid[1] != 0) {
+            if (!perimeter_segment(a.g, j, i0, h.mid[1]) && h.mid[1] != 0) {
                 const Idx x = lane_idx(h, lane);
                 own[r] = x.src[0];
                 // A segment whose 3x3 neighbourhood is all wet (a scalar,
@@ -893,11 +904,22 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
             const int i = (int)row0 + 64 * r + lane;
             collide_shifted<T>(a, p[r], -1, 0, i, j, own[r], out[r]);
         }
+        const uint32_t b0 = (uint32_t)own[0] * 4u, b1 = (uint32_t)own[1] * 4u;
+        if ((lines & 3u) == 3u) {
+            asm volatile("" ::: "memory");
 #pragma unroll
-        for (int d = 0; d < 9; ++d)
+            for (int d = 0; d < 9; ++d) {
+                __builtin_nontemporal_store(out[0][d], (T *)((char *)a.fout_plane[d] + b0));
+                __builtin_nontemporal_store(out[1][d], (T *)((char *)a.fout_plane[d] + b1));
+            }
+            asm volatile("" ::: "memory");
+        } else {
 #pragma unroll
-            for (int r = 0; r < R; ++r)
-                if (act[r]) store_plane(a, d, own[r], out[r][d], (lines >> r) & 1);
+            for (int d = 0; d < 9; ++d) {
+                if (act[0]) st((T *)a.fout_plane[d], b0, out[0][d]);
+                if (act[1]) st((T *)a.fout_plane[d], b1, out[1][d]);
+            }
+        }
     } else {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -906,22 +928,39 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
             T out[9];
             if constexpr (sizeof(T) == 8) collide_f64(a, p[r], -1, 0, i, j, out);
             else collide_shifted<T>(a, p[r], -1, 0, i, j, own[r], out);
-            const bool whole = (lines >> r) & 1;
-#pragma unroll
-            for (int d = 0; d < 9; ++d) store_plane(a, d, own[r], out[d], whole);
+            if ((lines >> r) & 1) store9_stream(a, own[r], out);
+            else store9(a, own[r], out);
         }
     }
-    if (general) {
-#pragma unroll 1
-        for (int r = 0; r < R; ++r) {
-            if (!((general >> r) & 1)) continue;
-            const int64_t i0 = row0 + 64 * r;
-            const Nbhd h = load_nbhd(a.g, j, i0);
-            const int i = (int)i0 + lane;
-            if constexpr (sizeof(T) == 8) node_general_f64<GATHER>(a, i, j, lane, h);
-            else node_general_shifted<T, GATHER>(a, i, j, lane, h);
-        }
+}
+
+// The perimeter share of a gathering step, item w of 2 S + 2 (NC - 2)
+// (S = 64-row segments per column): items [0, S) are the segments of
+// column 0, [S, 2S) those of column NC-1, and each interior column has two
+// items, its first and its last segment.  Only segments holding a wet
+// perimeter node do work (the same test the step kernel skips them by).
+template <typename T>
+__device__ __forceinline__ void perimeter_wave(const StepArgs &a, int64_t w, int lane)
+{
+    const int64_t S = a.g.ldr / 64;
+    int j;
+    int64_t k;
+    if (w < 2 * S) {
+        j = w < S ? 0 : a.g.nc - 1;
+        k = w < S ? w : w - S;
+    } else {
+        const int64_t v = w - 2 * S;
+        j = 1 + (int)(v >> 1);
+        if (j >= a.g.nc - 1) return;
+        k = (v & 1) ? S - 1 : 0;
+        if ((v & 1) && S == 1) return;  // one segment: item 0 covers it
     }
+    const int64_t i0 = 64 * k;
+    const uint64_t mid = a.g.wet[(int64_t)(j + 1) * a.g.wl + 1 + k];
+    if (!perimeter_segment(a.g, j, i0, __builtin_amdgcn_readfirstlane((uint32_t)mid) |
+                                            ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(mid >> 32)) << 32)))
+        return;
+    general_segment<T, true>(a, j, i0, lane);
 }
 
 // The time-step kernel of one lattice: wave w of the (XCD-remapped) grid
@@ -937,6 +976,35 @@ __global__ __launch_bounds__(kBlockRows) void step_kernel(StepArgs a)
     if (j >= a.g.nc) return;
     const int64_t row0 = (gw - (int64_t)j * a.wpc) * (64 * R);
     step_wave<T, GATHER, R>(a, j, row0, lane);
+}
+
+// The perimeter share of a gathering step of one lattice (perimeter_wave):
+// one wave per item, 2 S + 2 (NC - 2) items.
+template <typename T>
+__global__ __launch_bounds__(kBlockRows) void perimeter_kernel(StepArgs a)
+{
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t w = (int64_t)blockIdx.x * (kBlockRows / 64) + wave;
+    const int64_t items = 2 * (a.g.ldr / 64) + 2 * (int64_t)(a.g.nc - 2);
+    if (w >= items) return;
+    perimeter_wave<T>(a, w, threadIdx.x & 63);
+}
+
+// The perimeter share of a gathering step of a batch: one wave per entry
+// (sample, column, first row) of the perimeter table, which lists every
+// 64-row segment holding a wet perimeter node.
+template <typename T>
+__global__ __launch_bounds__(kBlockRows) void batch_perimeter_kernel(BatchArgs bt)
+{
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t t = (int64_t)blockIdx.x * (kBlockRows / 64) + wave;
+    if (t >= bt.nptiles) return;
+    const int32_t *tile = bt.ptiles + 3 * t;
+    const int s = __builtin_amdgcn_readfirstlane(tile[0]);
+    const int j = __builtin_amdgcn_readfirstlane(tile[1]);
+    const int row0 = __builtin_amdgcn_readfirstlane(tile[2]);
+    const StepArgs *__restrict__ a = bt.args + 2 * (int64_t)s + bt.parity;
+    general_segment<T, true>(*a, j, row0, threadIdx.x & 63);
 }
 
 // The time-step kernel of a batch of independent lattices: a persistent
@@ -1286,7 +1354,11 @@ hipError_t launch_step_t(bool gather, StepArgs a, hipStream_t s)
 {
     a.wpc = (int32_t)((a.g.ldr / 64 + R - 1) / R);
     const int64_t nb = step_blocks(a, R);
-    return gather ? launch(step_kernel<T, true, R>, nb, s, a) : launch(step_kernel<T, false, R>, nb, s, a);
+    if (!gather) return launch(step_kernel<T, false, R>, nb, s, a);
+    hipError_t e = launch(step_kernel<T, true, R>, nb, s, a);
+    if (e != hipSuccess) return e;
+    const int64_t items = 2 * (a.g.ldr / 64) + 2 * (a.g.nc - 2);
+    return launch(perimeter_kernel<T>, (items + kBlockRows / 64 - 1) / (kBlockRows / 64), s, a);
 }
 
 template <typename T>
@@ -1304,8 +1376,15 @@ template <typename T, int R>
 hipError_t launch_batch_t(bool gather, const BatchArgs &b, int64_t blocks, hipStream_t s)
 {
     dim3 grid((unsigned)blocks), block(kBlockRows);
-    if (gather) hipLaunchKernelGGL((batch_kernel<T, true, R>), grid, block, 0, s, b);
-    else hipLaunchKernelGGL((batch_kernel<T, false, R>), grid, block, 0, s, b);
+    if (!gather) {
+        hipLaunchKernelGGL((batch_kernel<T, false, R>), grid, block, 0, s, b);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL((batch_kernel<T, true, R>), grid, block, 0, s, b);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || b.nptiles == 0) return e;
+    const int64_t pb = (b.nptiles + kBlockRows / 64 - 1) / (kBlockRows / 64);
+    hipLaunchKernelGGL((batch_perimeter_kernel<T>), dim3((unsigned)pb), block, 0, s, b);
     return hipGetLastError();
 }
 
