@@ -697,9 +697,8 @@ __device__ __forceinline__ void moments_shifted(const float *g, Moments &m)
 // hardware reciprocal's error is far below FP32 resolution of the result.
 // TRT on pairs of n = geq - g:  o_d = g_d + ca (n_d + n_q) + cb (n_d - n_q)
 // (BGK: ca = cb = omega / 2).
-template <typename T>
 __device__ __forceinline__ void collide_shifted(const StepArgs &a, const float *g, int policy, int64_t e, int i, int j,
-                                                int64_t n, T *raw)
+                                                float *o)
 {
     Moments m;
     moments_shifted(g, m);
@@ -735,7 +734,7 @@ __device__ __forceinline__ void collide_shifted(const StepArgs &a, const float *
     // c.j of the first direction of each pair: 1 (+x), 2 (+y), 5 (+x+y), 6 (-x+y)
     const float cj[4] = {jx, jy, jx + jy, jy - jx};
     constexpr int kD[4] = {1, 2, 5, 6};
-    float o[9], seq_sum;
+    float seq_sum;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int d = kD[k], q = kOpp[d];
@@ -752,13 +751,23 @@ __device__ __forceinline__ void collide_shifted(const StepArgs &a, const float *
     }
     const float geq0 = drho - seq_sum;
     o[0] = __builtin_fmaf(2.f * ca, geq0 - g[0], g[0]);
+}
+
+// Converts the post-collision shifted populations to the storage type and
+// stores them (whole_lines: non-temporal, see store9_stream).  FP16 rounds
+// stochastically unless the context asked for round-to-nearest: 13 random
+// bits are added below the FP16 cut of the FP32 value, which is then
+// truncated (sign-magnitude, so the same for both signs).  The bits come
+// from one hash per node of its index and its new rest population, a
+// different window per direction.  Each rounding mode stores in its own
+// branch so the uniform flag stays a branch instead of both conversions
+// being computed and selected.
+template <typename T>
+__device__ __forceinline__ void encode_store(const StepArgs &a, int64_t n, const float *o, bool whole_lines)
+{
+    T raw[9];
     if constexpr (sizeof(T) == 2) {
         if (a.f16_sr) {
-            // Stochastic rounding to FP16: add 13 random bits below the
-            // FP16 cut of the FP32 value and truncate (sign-magnitude, so
-            // the same for both signs).  The bits come from one hash per
-            // node of its index and its new rest population, a different
-            // window per direction.
             uint32_t h = (uint32_t)n * 0x9E3779B9u ^ __float_as_uint(o[0]);
             h ^= h >> 16;
             h *= 0x85EBCA6Bu;
@@ -771,20 +780,24 @@ __device__ __forceinline__ void collide_shifted(const StepArgs &a, const float *
                 const uint32_t b = (__float_as_uint(o[d] * a.scale) + r) & ~0x1FFFu;
                 raw[d] = (T)__uint_as_float(b);
             }
+            if (whole_lines) store9_stream(a, n, raw);
+            else store9(a, n, raw);
             return;
         }
     }
 #pragma unroll
     for (int d = 0; d < 9; ++d) raw[d] = from_g<T>(o[d], a.scale);
+    if (whole_lines) store9_stream(a, n, raw);
+    else store9(a, n, raw);
 }
 
 template <typename T>
 __device__ __forceinline__ void collide_store_shifted(const StepArgs &a, const float *g, int policy, int64_t e, int i,
                                                       int j, int64_t n)
 {
-    T raw[9];
-    collide_shifted<T>(a, g, policy, e, i, j, n, raw);
-    store9(a, n, raw);
+    float o[9];
+    collide_shifted(a, g, policy, e, i, j, o);
+    encode_store<T>(a, n, o, false);
 }
 
 template <typename T, bool GATHER>
@@ -902,7 +915,7 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
         for (int r = 0; r < R; ++r) {
             if (!act[r]) continue;
             const int i = (int)row0 + 64 * r + lane;
-            collide_shifted<T>(a, p[r], -1, 0, i, j, own[r], out[r]);
+            collide_shifted(a, p[r], -1, 0, i, j, out[r]);
         }
         const uint32_t b0 = (uint32_t)own[0] * 4u, b1 = (uint32_t)own[1] * 4u;
         if ((lines & 3u) == 3u) {
@@ -925,11 +938,16 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
         for (int r = 0; r < R; ++r) {
             if (!act[r]) continue;
             const int i = (int)row0 + 64 * r + lane;
-            T out[9];
-            if constexpr (sizeof(T) == 8) collide_f64(a, p[r], -1, 0, i, j, out);
-            else collide_shifted<T>(a, p[r], -1, 0, i, j, own[r], out);
-            if ((lines >> r) & 1) store9_stream(a, own[r], out);
-            else store9(a, own[r], out);
+            if constexpr (sizeof(T) == 8) {
+                double out[9];
+                collide_f64(a, p[r], -1, 0, i, j, out);
+                if ((lines >> r) & 1) store9_stream(a, own[r], out);
+                else store9(a, own[r], out);
+            } else {
+                float out[9];
+                collide_shifted(a, p[r], -1, 0, i, j, out);
+                encode_store<T>(a, own[r], out, (lines >> r) & 1);
+            }
         }
     }
 }
