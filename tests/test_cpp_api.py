@@ -75,3 +75,32 @@ def test_cpp_driver_matches_oracle(tmp_path):
     ld_err = float(np.max(np.abs(cpp_ld - np.array(ld))))
     print(f"C++ driver vs oracle: max|d(lift, drag)| = {ld_err:.3e}", flush=True)
     assert ld_err <= 1e-12
+
+
+def test_readers_and_writer_match_the_reference_formats(tmp_path):
+    """The C++ lattice/obstacle readers and the result writer (host only):
+    lid-lattice.txt (the reference's 129 x 129 fixture) reads as the cavity
+    plane of lbm_geometry_cavity; wing1.txt marks the same nodes the Python
+    reader of the parity tests marks; the writer emits one text row per
+    lattice row of u_x / u_y."""
+    exe = os.path.join(CPP_DIR, "io_check")
+    if not os.path.exists(exe):
+        subprocess.run(["make", "-s", "-C", CPP_DIR, "io_check"], check=True)
+    golden = os.path.join(ROOT, "tests", "golden")
+    out_cls, out_obst, results = tmp_path / "cls.bin", tmp_path / "obst.bin", tmp_path / "results"
+    r = subprocess.run([exe, os.path.join(golden, "lid-lattice.txt"), str(out_cls), os.path.join(golden, "wing1.txt"),
+                        "100", "400", str(out_obst), str(results)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert "dims 129 129" in r.stdout
+    cls = np.fromfile(out_cls, dtype=np.uint8).reshape(129, 129)      # (NC, NR) = ColMajor plane
+    assert np.array_equal(cls, L.geometry_cavity(129, 129))
+    pts = np.loadtxt(os.path.join(golden, "wing1.txt"), skiprows=1, dtype=np.int64)
+    expect = np.zeros((400, 100), dtype=np.uint8)
+    expect[pts[:, 1], pts[:, 0]] = L.OBSTACLE
+    assert np.array_equal(np.fromfile(out_obst, dtype=np.uint8).reshape(400, 100), expect)
+    assert f"obstacles {len(pts)}" in r.stdout
+    ux = (results / "output-7u_x.txt").read_text().splitlines()
+    uy = (results / "output-7u_y.txt").read_text().splitlines()
+    assert len(ux) == 3 and len(uy) == 3
+    assert [float(x) for x in ux[1].split()] == [0.25 * (4 + j) for j in range(4)]
+    assert [float(x) for x in uy[2].split()] == [-(2.0 + j) for j in range(4)]
