@@ -94,16 +94,23 @@ def test_graph_replays_equal_eager_launches():
 def test_fp64_ensemble_matches_oracle():
     wls = W.ensemble(n_samples=3, seed=29, sizes=(128, 192))
     steps = 300
-    say(f"case FP64 ensemble vs oracle, {steps} steps")
+    say(f"case FP64 ensemble vs oracle (populations and permeability), {steps} steps")
     lats = [w.make_lattice(L.F64) for w in wls]
     with L.Batch(lats) as b:
         b.step(steps)
-    for w, lat in zip(wls, lats):
+        red = b.reduce()
+    nu = (0.8 - 0.5) / 3.0
+    for s, (w, lat) in enumerate(zip(wls, lats)):
         orc = oracle_from(w, threads=8)
         orc.step(steps)
         err = max_abs_wet(lat.populations(), orc.populations(), w.node_class)
-        say(f"  {w.name}: max|dP| = {err:.3e}")
+        # per-sample Darcy permeability: ensemble FP64 reduction on the GPU
+        # against the oracle's own FP64 reduction of the same sample
+        k_gpu = W.darcy_permeability(red[s], nu, w.nc - 1, 0.001, w.nr * w.nc)
+        k_orc = W.darcy_permeability(orc.reduce(), nu, w.nc - 1, 0.001, w.nr * w.nc)
+        say(f"  {w.name}: max|dP| = {err:.3e}, permeability {k_gpu:.10e} (oracle {k_orc:.10e})")
         assert err <= 1e-12
+        assert abs(k_gpu - k_orc) <= 1e-10 * abs(k_orc)
         orc.close()
         lat.close()
 
