@@ -18,7 +18,10 @@ reductions) are checked against an FP64 ensemble.
 
 MLUPS = wet nodes * steps / time / 1e6.  Algorithmic HBM bytes per step =
 wet nodes * 18 * sizeof(storage) + NR*NC/8 (bit-packed wet mask); the
-roofline fraction divides the achieved rate by MI355X's 8 TB/s.  The step
+roofline fraction divides the achieved rate by MI355X's 8 TB/s; the rate of
+the HIP runtime's device-to-device copy over one population buffer's volume
+is measured in the same run and reported beside it as the practical
+ceiling of a plain stream (copy_ceiling, frac_of_copy).  The step
 kernel is the only kernel in the timed region, so its average duration is
 the event-timed region divided by the number of steps.
 
@@ -79,6 +82,35 @@ class HipEvents:
         self.hip.hipEventDestroy(a)
         self.hip.hipEventDestroy(b)
         return ms.value
+
+
+def copy_ceiling(ev, nbytes, reps=20):
+    """Device-to-device hipMemcpyAsync of nbytes (the runtime's own copy
+    kernel) timed with events: the practical HBM rate of a plain stream,
+    read + write bytes per second."""
+    hip = ev.hip
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    hip.hipMemsetAsync.argtypes = [C.c_void_p, C.c_int, C.c_size_t, C.c_void_p]
+    hip.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+    hip.hipDeviceSynchronize.argtypes = []
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), nbytes) == 0 and hip.hipMalloc(C.byref(dst), nbytes) == 0
+    try:
+        assert hip.hipMemsetAsync(src, 1, nbytes, None) == 0 and hip.hipMemsetAsync(dst, 0, nbytes, None) == 0
+        for _ in range(3):
+            assert hip.hipMemcpyAsync(dst, src, nbytes, 3, None) == 0   # 3 = hipMemcpyDeviceToDevice
+        assert hip.hipDeviceSynchronize() == 0
+
+        def run():
+            for _ in range(reps):
+                assert hip.hipMemcpyAsync(dst, src, nbytes, 3, None) == 0
+        ms = ev.time_ms(None, run)
+    finally:
+        hip.hipDeviceSynchronize()
+        hip.hipFree(src)
+        hip.hipFree(dst)
+    return dict(kind="hipMemcpyAsync device-to-device", bytes=nbytes, gbs=2 * nbytes * reps / (ms / 1e3) / 1e9)
 
 
 def build_workload(name):
@@ -276,13 +308,18 @@ def main():
     log(f"workload {wl.name}: {wl.nr}x{wl.nc}, wet {wl.wet_count()}, info {wl.info} "
         f"(built in {time.perf_counter() - t0:.1f}s)")
     head = args.storage
+    # the copy ceiling over the FP32 step's volume (one population buffer)
+    ceiling = copy_ceiling(ev, W.algorithmic_bytes(wl, L.F32_SHIFTED) // 2)
+    log(f"  device copy ceiling: {ceiling['gbs']:.0f} GB/s ({ceiling['bytes'] / 1e9:.2f} GB copies)")
     order = [head] if args.only else [head] + [s for s in VARIANTS if s != head]
     results, lats = {}, {}
     for s in order:
         lat, r = time_storage(wl, s, args.steps, args.warmup, ev)
+        r["frac_of_copy"] = r["achieved_gbs"] / ceiling["gbs"]
         results[s] = r
         log(f"  {r['storage']}: {r['mlups']:.1f} MLUPS, {r['ms_per_step']:.4f} ms/step, "
-            f"{r['achieved_gbs']:.0f} GB/s = {100 * r['frac']:.1f}% of 8 TB/s")
+            f"{r['achieved_gbs']:.0f} GB/s = {100 * r['frac']:.1f}% of 8 TB/s, "
+            f"{100 * r['frac_of_copy']:.0f}% of the copy ceiling")
         lats[s] = lat
     errors = None
     error_steps = wl.steps if args.error_steps is None else args.error_steps
@@ -321,6 +358,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(r["frac"], 4), "traffic": r["traffic_bytes_per_step"]},
         "cpu_baseline": cpu,
+        "copy_ceiling": {k: (round(v, 1) if isinstance(v, float) else v) for k, v in ceiling.items()},
         "storages": {s: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in x.items()}
                      for s, x in results.items()},
         "error_vs_f64": errors,

@@ -73,6 +73,13 @@ __device__ __forceinline__ float mul16(float x) { return __builtin_fmaf(k16h, x,
 // Boundary types of the reference's btypes enum (aliases.hpp:28-38).
 enum : int { BT_NONE = 0, BT_TOP, BT_RIGHT, BT_BOTTOM, BT_LEFT, BT_CORNER };
 
+// Global (1) and constant (4) address spaces: accesses through the first
+// are vector global loads/stores with a scalar base; reads through the
+// second, of data no kernel writes while it runs, are scalar loads where
+// the address is wave-uniform.
+#define LBM_GLOBAL __attribute__((address_space(1)))
+#define LBM_CONST __attribute__((address_space(4)))
+
 // Spreads the blocks of a launch so that each XCD (blocks are dealt
 // round-robin to the 8 XCDs) gets one contiguous run of the column-major
 // tile order: neighbouring row segments of a column then share an L2.
@@ -140,11 +147,13 @@ __device__ __forceinline__ Nbhd load_nbhd(const Geometry &g, int j, int64_t i0)
 {
     Nbhd h;
     const int64_t k = i0 >> 6;
-    const uint64_t *wbase = g.wet + (int64_t)j * g.wl + k;  // column j-1, word k-1
-    const int32_t *sbase = g.sb + (int64_t)j * g.wl + k + 1; // column j-1, word k
+    // The mask and the segment bases never change while a kernel runs
+    // (constant address space: scalar loads from any kernel).
+    const LBM_CONST uint64_t *wbase = (const LBM_CONST uint64_t *)g.wet + (int64_t)j * g.wl + k;  // column j-1, word k-1
+    const LBM_CONST int32_t *sbase = (const LBM_CONST int32_t *)g.sb + (int64_t)j * g.wl + k + 1; // column j-1, word k
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const uint64_t *col = wbase + c * g.wl;
+        const LBM_CONST uint64_t *col = wbase + c * g.wl;
         uint64_t w0 = col[0], w1 = col[1], w2 = col[2];
         h.up[c] = (w1 << 1) | (w0 >> 63);
         h.mid[c] = w1;
@@ -217,17 +226,33 @@ __device__ __forceinline__ Idx lane_idx(const Nbhd &h, int lane)
 // Element access through a wave-uniform base pointer and a 32-bit byte
 // offset, the addressing form global loads/stores take natively (scalar
 // base + vector offset).  The context only accepts lattices whose
-// population offsets (up to three planes) stay below 2^32 bytes.
+// population offsets (up to three planes) stay below 2^32 bytes.  The
+// accesses go through global-address-space pointers: a plane pointer read
+// from memory (the batch kernels' per-sample arguments) is otherwise a
+// generic pointer, and generic accesses become flat instructions with
+// 64-bit vector addresses.
+template <typename T>
+__device__ __forceinline__ const LBM_GLOBAL T *gaddr(const void *base, uint32_t byte_off)
+{
+    return (const LBM_GLOBAL T *)((const LBM_GLOBAL char *)base + byte_off);
+}
+
+template <typename T>
+__device__ __forceinline__ LBM_GLOBAL T *gaddr(void *base, uint32_t byte_off)
+{
+    return (LBM_GLOBAL T *)((LBM_GLOBAL char *)base + byte_off);
+}
+
 template <typename T>
 __device__ __forceinline__ T ld(const T *__restrict__ base, uint32_t byte_off)
 {
-    return *(const T *)((const char *)base + byte_off);
+    return *gaddr<T>((const void *)base, byte_off);
 }
 
 template <typename T>
 __device__ __forceinline__ void st(T *__restrict__ base, uint32_t byte_off, T v)
 {
-    *(T *)((char *)base + byte_off) = v;
+    *gaddr<T>((void *)base, byte_off) = v;
 }
 
 // Streams the 9 populations of an interior wet node: one load per
@@ -305,10 +330,15 @@ __device__ __forceinline__ void gather_edge(const StepArgs &a, const T *__restri
 }
 
 // Writes the 9 post-collision populations of the node with compact index n.
+// Each store helper takes its own opaque copy of the byte offset (empty
+// asm): the compiler then cannot hoist the nine plane addresses shared by
+// the plain and the streaming branch into 64-bit vector pointers, and keeps
+// every store in the scalar-base + 32-bit-offset form.
 template <typename T>
 __device__ __forceinline__ void store9(const StepArgs &a, int64_t n, const T *o)
 {
-    const uint32_t b = (uint32_t)n * (uint32_t)sizeof(T);
+    uint32_t b = (uint32_t)n * (uint32_t)sizeof(T);
+    asm volatile("" : "+v"(b));
 #pragma unroll
     for (int d = 0; d < 9; ++d) st((T *)a.fout_plane[d], b, o[d]);
 }
@@ -322,10 +352,10 @@ __device__ __forceinline__ void store9(const StepArgs &a, int64_t n, const T *o)
 template <typename T>
 __device__ __forceinline__ void store9_stream(const StepArgs &a, int64_t n, const T *o)
 {
-    const uint32_t b = (uint32_t)n * (uint32_t)sizeof(T);
-    asm volatile("" ::: "memory");
+    uint32_t b = (uint32_t)n * (uint32_t)sizeof(T);
+    asm volatile("" : "+v"(b)::"memory");
 #pragma unroll
-    for (int d = 0; d < 9; ++d) __builtin_nontemporal_store(o[d], (T *)((char *)a.fout_plane[d] + b));
+    for (int d = 0; d < 9; ++d) __builtin_nontemporal_store(o[d], gaddr<T>(a.fout_plane[d], b));
     asm volatile("" ::: "memory");
 }
 
@@ -487,9 +517,10 @@ __device__ __forceinline__ void collide_f64(const StepArgs &a, const double *p, 
     }
     if (a.macro) {
         int64_t q = (int64_t)j * a.g.nr + i, N = (int64_t)a.g.nr * a.g.nc;
-        a.macro[q] = rho;
-        a.macro[N + q] = ux;
-        a.macro[2 * N + q] = uy;
+        LBM_GLOBAL double *mq = (LBM_GLOBAL double *)a.macro + q;
+        mq[0] = rho;
+        mq[N] = ux;
+        mq[2 * N] = uy;
     }
     double eq[9];
     feq64(rho, ux, uy, eq);
@@ -724,9 +755,10 @@ __device__ __forceinline__ void collide_shifted(const StepArgs &a, const float *
     }
     if (a.macro) {
         int64_t q = (int64_t)j * a.g.nr + i, N = (int64_t)a.g.nr * a.g.nc;
-        a.macro[q] = rho_out;
-        a.macro[N + q] = ux_out;
-        a.macro[2 * N + q] = uy_out;
+        LBM_GLOBAL double *mq = (LBM_GLOBAL double *)a.macro + q;
+        mq[0] = rho_out;
+        mq[N] = ux_out;
+        mq[2 * N] = uy_out;
     }
     const float rinv = __builtin_amdgcn_rcpf(1.f + drho);
     const float j15 = 1.5f * __builtin_fmaf(jx, jx, jy * jy);
@@ -917,16 +949,17 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
             const int i = (int)row0 + 64 * r + lane;
             collide_shifted(a, p[r], -1, 0, i, j, out[r]);
         }
-        const uint32_t b0 = (uint32_t)own[0] * 4u, b1 = (uint32_t)own[1] * 4u;
+        uint32_t b0 = (uint32_t)own[0] * 4u, b1 = (uint32_t)own[1] * 4u;
         if ((lines & 3u) == 3u) {
-            asm volatile("" ::: "memory");
+            asm volatile("" : "+v"(b0), "+v"(b1)::"memory");
 #pragma unroll
             for (int d = 0; d < 9; ++d) {
-                __builtin_nontemporal_store(out[0][d], (T *)((char *)a.fout_plane[d] + b0));
-                __builtin_nontemporal_store(out[1][d], (T *)((char *)a.fout_plane[d] + b1));
+                __builtin_nontemporal_store(out[0][d], gaddr<T>(a.fout_plane[d], b0));
+                __builtin_nontemporal_store(out[1][d], gaddr<T>(a.fout_plane[d], b1));
             }
             asm volatile("" ::: "memory");
         } else {
+            asm volatile("" : "+v"(b0), "+v"(b1));
 #pragma unroll
             for (int d = 0; d < 9; ++d) {
                 if (act[0]) st((T *)a.fout_plane[d], b0, out[0][d]);
@@ -1008,6 +1041,32 @@ __global__ __launch_bounds__(kBlockRows) void perimeter_kernel(StepArgs a)
     perimeter_wave<T>(a, w, threadIdx.x & 63);
 }
 
+// Entry t of a batch tile table (sample, column, first row) and the step
+// arguments of sample s, read through the constant address space: neither
+// changes while a kernel runs, so the wave-uniform reads become scalar
+// loads and the arguments live in scalar registers.
+struct TileRef {
+    int s, j, row0;
+};
+
+__device__ __forceinline__ TileRef tile_at(const int32_t *table, int64_t t)
+{
+    const LBM_CONST int32_t *p = (const LBM_CONST int32_t *)table + 3 * t;
+    return {__builtin_amdgcn_readfirstlane(p[0]), __builtin_amdgcn_readfirstlane(p[1]),
+            __builtin_amdgcn_readfirstlane(p[2])};
+}
+
+__device__ __forceinline__ StepArgs args_at(const BatchArgs &bt, int s)
+{
+    static_assert(sizeof(StepArgs) % 4 == 0, "word copy");
+    const LBM_CONST uint32_t *src = (const LBM_CONST uint32_t *)(bt.args + 2 * (int64_t)s + bt.parity);
+    StepArgs a;
+    uint32_t *dst = (uint32_t *)&a;
+#pragma unroll
+    for (unsigned k = 0; k < sizeof(StepArgs) / 4; ++k) dst[k] = src[k];
+    return a;
+}
+
 // The perimeter share of a gathering step of a batch: one wave per entry
 // (sample, column, first row) of the perimeter table, which lists every
 // 64-row segment holding a wet perimeter node.
@@ -1017,12 +1076,9 @@ __global__ __launch_bounds__(kBlockRows) void batch_perimeter_kernel(BatchArgs b
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t t = (int64_t)blockIdx.x * (kBlockRows / 64) + wave;
     if (t >= bt.nptiles) return;
-    const int32_t *tile = bt.ptiles + 3 * t;
-    const int s = __builtin_amdgcn_readfirstlane(tile[0]);
-    const int j = __builtin_amdgcn_readfirstlane(tile[1]);
-    const int row0 = __builtin_amdgcn_readfirstlane(tile[2]);
-    const StepArgs *__restrict__ a = bt.args + 2 * (int64_t)s + bt.parity;
-    general_segment<T, true>(*a, j, row0, threadIdx.x & 63);
+    const TileRef tr = tile_at(bt.ptiles, t);
+    const StepArgs a = args_at(bt, tr.s);
+    general_segment<T, true>(a, tr.j, tr.row0, threadIdx.x & 63);
 }
 
 // The time-step kernel of a batch of independent lattices: a persistent
@@ -1042,12 +1098,9 @@ __global__ __launch_bounds__(kBlockRows) void batch_kernel(BatchArgs bt)
         // one wave per tile; each XCD takes one contiguous run of the table
         const int64_t t = xcd_remap(blockIdx.x, gridDim.x) * (kBlockRows / 64) + wave;
         if (t >= bt.ntiles) return;
-        const int32_t *tile = bt.tiles + 3 * t;
-        const int s = __builtin_amdgcn_readfirstlane(tile[0]);
-        const int j = __builtin_amdgcn_readfirstlane(tile[1]);
-        const int row0 = __builtin_amdgcn_readfirstlane(tile[2]);
-        const StepArgs *__restrict__ a = bt.args + 2 * (int64_t)s + bt.parity;
-        step_wave<T, GATHER, R>(*a, j, row0, lane);
+        const TileRef tr = tile_at(bt.tiles, t);
+        const StepArgs a = args_at(bt, tr.s);
+        step_wave<T, GATHER, R>(a, tr.j, tr.row0, lane);
         return;
     }
     const int64_t xcd = blockIdx.x & 7;
@@ -1059,12 +1112,9 @@ __global__ __launch_bounds__(kBlockRows) void batch_kernel(BatchArgs bt)
     const int64_t last = first + chunk < bt.ntiles ? first + chunk : bt.ntiles;
 #pragma unroll 1
     for (int64_t t = first + local_wave; t < last; t += stride) {
-        const int32_t *tile = bt.tiles + 3 * t;
-        const int s = __builtin_amdgcn_readfirstlane(tile[0]);
-        const int j = __builtin_amdgcn_readfirstlane(tile[1]);
-        const int row0 = __builtin_amdgcn_readfirstlane(tile[2]);
-        const StepArgs *__restrict__ a = bt.args + 2 * (int64_t)s + bt.parity;
-        step_wave<T, GATHER, R>(*a, j, row0, lane);
+        const TileRef tr = tile_at(bt.tiles, t);
+        const StepArgs a = args_at(bt, tr.s);
+        step_wave<T, GATHER, R>(a, tr.j, tr.row0, lane);
     }
 }
 
@@ -1174,11 +1224,9 @@ __global__ __launch_bounds__(kBlockRows) void batch_partials(BatchArgs bt, doubl
 #pragma unroll 1
     for (int64_t t = (int64_t)blockIdx.x * (kBlockRows / 64) + wave; t < bt.ntiles;
          t += (int64_t)gridDim.x * (kBlockRows / 64)) {
-        const int32_t *tile = bt.tiles + 3 * t;
-        const int s = __builtin_amdgcn_readfirstlane(tile[0]);
-        const int j = __builtin_amdgcn_readfirstlane(tile[1]);
-        const int row0 = __builtin_amdgcn_readfirstlane(tile[2]);
-        const StepArgs &a = bt.args[2 * (int64_t)s + bt.parity];
+        const TileRef tr = tile_at(bt.tiles, t);
+        const int j = tr.j, row0 = tr.row0;
+        const StepArgs a = args_at(bt, tr.s);
         double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll 1
         for (int r = 0; r < R; ++r) {
