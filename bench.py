@@ -19,9 +19,9 @@ reductions) are checked against an FP64 ensemble.
 MLUPS = wet nodes * steps / time / 1e6.  Algorithmic HBM bytes per step =
 wet nodes * 18 * sizeof(storage) + NR*NC/8 (bit-packed wet mask); the
 roofline fraction divides the achieved rate by MI355X's 8 TB/s; the rate of
-the HIP runtime's device-to-device copy over one population buffer's volume
-is measured in the same run and reported beside it as the practical
-ceiling of a plain stream (copy_ceiling, frac_of_copy).  The step
+a plain 16-byte-per-lane copy kernel over one FP32 population buffer's
+volume is measured in the same run and reported beside it as the practical
+ceiling of a stream (copy_ceiling, frac_of_copy).  The step
 kernel is the only kernel in the timed region, so its average duration is
 the event-timed region divided by the number of steps.
 
@@ -84,33 +84,12 @@ class HipEvents:
         return ms.value
 
 
-def copy_ceiling(ev, nbytes, reps=20):
-    """Device-to-device hipMemcpyAsync of nbytes (the runtime's own copy
-    kernel) timed with events: the practical HBM rate of a plain stream,
-    read + write bytes per second."""
-    hip = ev.hip
-    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
-    hip.hipFree.argtypes = [C.c_void_p]
-    hip.hipMemsetAsync.argtypes = [C.c_void_p, C.c_int, C.c_size_t, C.c_void_p]
-    hip.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
-    hip.hipDeviceSynchronize.argtypes = []
-    src, dst = C.c_void_p(), C.c_void_p()
-    assert hip.hipMalloc(C.byref(src), nbytes) == 0 and hip.hipMalloc(C.byref(dst), nbytes) == 0
-    try:
-        assert hip.hipMemsetAsync(src, 1, nbytes, None) == 0 and hip.hipMemsetAsync(dst, 0, nbytes, None) == 0
-        for _ in range(3):
-            assert hip.hipMemcpyAsync(dst, src, nbytes, 3, None) == 0   # 3 = hipMemcpyDeviceToDevice
-        assert hip.hipDeviceSynchronize() == 0
-
-        def run():
-            for _ in range(reps):
-                assert hip.hipMemcpyAsync(dst, src, nbytes, 3, None) == 0
-        ms = ev.time_ms(None, run)
-    finally:
-        hip.hipDeviceSynchronize()
-        hip.hipFree(src)
-        hip.hipFree(dst)
-    return dict(kind="hipMemcpyAsync device-to-device", bytes=nbytes, gbs=2 * nbytes * reps / (ms / 1e3) / 1e9)
+def copy_ceiling(nbytes, reps=20):
+    """The practical HBM ceiling of a plain stream over nbytes: the library's
+    16-byte-per-lane copy kernel (lbm_copy_bandwidth), read + write bytes
+    per second."""
+    return dict(kind="copy kernel, 16-byte lanes (lbm_copy_bandwidth)", bytes=int(nbytes),
+                gbs=L.copy_bandwidth(nbytes, reps))
 
 
 def build_workload(name):
@@ -259,8 +238,11 @@ def bench_ensemble(args, ev):
         lats64 = [w.make_lattice(L.F64) for w in wls]
         b64 = L.Batch(lats64)
         total = args.ensemble_check_steps
-        b.step(total - lats[0].time, L.STEP_CHECK)
-        b64.step(total, L.STEP_CHECK)
+        t1 = time.perf_counter()
+        for name, batch, lat0 in (("f32", b, lats[0]), ("f64", b64, lats64[0])):
+            while lat0.time < total:
+                batch.step(min(2500, total - lat0.time), L.STEP_CHECK)
+                log(f"    ensemble {name}: step {lat0.time} of {total} ({time.perf_counter() - t1:.1f}s)")
         red32, red64 = b.reduce(), b64.reduce()
         nu = (0.8 - 0.5) / 3.0
         k32 = np.array([W.darcy_permeability(red32[i], nu, w.nc - 1, 0.001, w.nr * w.nc) for i, w in enumerate(wls)])
@@ -293,7 +275,7 @@ def main():
                     help="steps after which the shifted-storage errors are stated (default: the workload's run "
                          "length, 20000 for porous8192; 0 = skip)")
     ap.add_argument("--ensemble-samples", type=int, default=512)
-    ap.add_argument("--ensemble-check-steps", type=int, default=2000,
+    ap.add_argument("--ensemble-check-steps", type=int, default=10000,
                     help="steps after which the ensemble's FP32 permeabilities are checked against FP64 (0 = skip)")
     ap.add_argument("--no-ensemble", action="store_true")
     args = ap.parse_args()
@@ -309,7 +291,7 @@ def main():
         f"(built in {time.perf_counter() - t0:.1f}s)")
     head = args.storage
     # the copy ceiling over the FP32 step's volume (one population buffer)
-    ceiling = copy_ceiling(ev, W.algorithmic_bytes(wl, L.F32_SHIFTED) // 2)
+    ceiling = copy_ceiling(W.algorithmic_bytes(wl, L.F32_SHIFTED) // 2)
     log(f"  device copy ceiling: {ceiling['gbs']:.0f} GB/s ({ceiling['bytes'] / 1e9:.2f} GB copies)")
     order = [head] if args.only else [head] + [s for s in VARIANTS if s != head]
     results, lats = {}, {}

@@ -270,6 +270,18 @@ int lbm_batch_get_stream(lbm_batch *batch, void **stream);
 int lbm_batch_synchronize(lbm_batch *batch);
 
 /* ------------------------------------------------------------------ */
+/* Measurement (no reference counterpart)                              */
+/* ------------------------------------------------------------------ */
+
+/* The practical HBM ceiling of a plain stream on `device`: the best rate
+ * (read + write bytes per second / 1e9) of a 16-byte-per-lane copy kernel
+ * over `bytes` bytes (rounded down to 16), with and without streaming
+ * stores, each timed over `reps` launches.  Allocates and frees two
+ * buffers of that size.  Benchmarks set the step kernels' achieved
+ * bandwidth against it. */
+int lbm_copy_bandwidth(int32_t device, int64_t bytes, int32_t reps, double *gbs);
+
+/* ------------------------------------------------------------------ */
 /* Geometry helpers (host only; write a class plane, ColMajor NR*NC)   */
 /* ------------------------------------------------------------------ */
 

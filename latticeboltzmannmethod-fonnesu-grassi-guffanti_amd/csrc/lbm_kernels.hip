@@ -143,6 +143,11 @@ struct Nbhd {
     int32_t base[3];
 };
 
+// EAGER: every word and base is needed before the wave's first population
+// load; naming the loaded values in an empty asm statement keeps the
+// compiler from sinking the neighbour columns' loads below a test on the
+// middle column, which would cost the wave a second round trip to L2.
+template <bool EAGER = false>
 __device__ __forceinline__ Nbhd load_nbhd(const Geometry &g, int j, int64_t i0)
 {
     Nbhd h;
@@ -151,14 +156,23 @@ __device__ __forceinline__ Nbhd load_nbhd(const Geometry &g, int j, int64_t i0)
     // (constant address space: scalar loads from any kernel).
     const LBM_CONST uint64_t *wbase = (const LBM_CONST uint64_t *)g.wet + (int64_t)j * g.wl + k;  // column j-1, word k-1
     const LBM_CONST int32_t *sbase = (const LBM_CONST int32_t *)g.sb + (int64_t)j * g.wl + k + 1; // column j-1, word k
+    uint64_t w[3][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const LBM_CONST uint64_t *col = wbase + c * g.wl;
-        uint64_t w0 = col[0], w1 = col[1], w2 = col[2];
-        h.up[c] = (w1 << 1) | (w0 >> 63);
-        h.mid[c] = w1;
-        h.dn[c] = (w1 >> 1) | (w2 << 63);
+        w[c][0] = col[0];
+        w[c][1] = col[1];
+        w[c][2] = col[2];
         h.base[c] = sbase[c * g.wl];
+    }
+    if (EAGER)
+        asm volatile("" ::"s"(w[0][0]), "s"(w[0][1]), "s"(w[0][2]), "s"(w[1][0]), "s"(w[1][1]), "s"(w[1][2]),
+                     "s"(w[2][0]), "s"(w[2][1]), "s"(w[2][2]), "s"(h.base[0]), "s"(h.base[1]), "s"(h.base[2]));
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        h.up[c] = (w[c][1] << 1) | (w[c][0] >> 63);
+        h.mid[c] = w[c][1];
+        h.dn[c] = (w[c][1] >> 1) | (w[c][2] << 63);
     }
     return h;
 }
@@ -714,8 +728,13 @@ __device__ __forceinline__ void moments_shifted(const float *g, Moments &m)
     m.a[0] = (float)a0; m.a[1] = (float)a1; m.a[2] = (float)a2; m.a[3] = (float)a3;
 }
 
-// Moments, equilibrium, collision and store of one wet node on shifted
-// populations g = f - w, FP32 arithmetic on exactly summed moments.
+// Moments, equilibrium and collision of one wet node on shifted
+// populations, FP32 arithmetic.  F16 = false (FP32 storage): g = f - w and
+// the moments are summed exactly in FP64 (moments_shifted).  F16 = true:
+// g = (f - w) S with S the FP16 storage scale, a power of two, so every
+// FP32 operation below gives exactly S (or S^2) times what it gives on
+// f - w and the scale never needs applying; the moments are summed in
+// FP32, whose rounding is far below the storage's 11-bit resolution.
 //
 // Per opposite pair (d, q = opp d), with c = c_d and j = rho u:
 //   geq_d + geq_q = 2 w (drho + (4.5 (c.j)^2 - 1.5 j.j) / rho)
@@ -725,33 +744,60 @@ __device__ __forceinline__ void moments_shifted(const float *g, Moments &m)
 // the node's mass up to one rounding.  The weights are applied in two
 // parts (hi + lo) so the equilibrium momentum is j to FP32 accuracy, not
 // (1 + 7.45e-9) j.  1/rho only scales the second-order term, where the
-// hardware reciprocal's error is far below FP32 resolution of the result.
+// hardware reciprocal's error is far below FP32 resolution of the result
+// (scaled form: 1 / (S rho) = 1 / (S + S drho)).
 // TRT on pairs of n = geq - g:  o_d = g_d + ca (n_d + n_q) + cb (n_d - n_q)
 // (BGK: ca = cb = omega / 2).
+template <bool F16>
 __device__ __forceinline__ void collide_shifted(const StepArgs &a, const float *g, int policy, int64_t e, int i, int j,
                                                 float *o)
 {
-    Moments m;
-    moments_shifted(g, m);
+    float ps[4], pa[4];  // pair sums / differences, pairs (1,3) (2,4) (5,7) (6,8)
     float drho, jx, jy;
+    double md, mx, my;   // moments of f - w for the macro output
+    if constexpr (F16) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int d = k < 2 ? k + 1 : k + 3;
+            ps[k] = g[d] + g[kOpp[d]];
+            pa[k] = g[d] - g[kOpp[d]];
+        }
+        drho = g[0] + ((ps[0] + ps[1]) + (ps[2] + ps[3]));
+        jx = pa[0] + (pa[2] - pa[3]);
+        jy = pa[1] + (pa[2] + pa[3]);
+        md = (double)drho * a.inv_scale;
+        mx = (double)jx * a.inv_scale;
+        my = (double)jy * a.inv_scale;
+    } else {
+        Moments m;
+        moments_shifted(g, m);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            ps[k] = m.s[k];
+            pa[k] = m.a[k];
+        }
+        md = m.drho;
+        mx = m.jx;
+        my = m.jy;
+        drho = (float)md;
+        jx = (float)mx;
+        jy = (float)my;
+    }
+    if (md != md) atomicOr(a.flag, 1);
     double rho_out, ux_out, uy_out;
     if (policy == kVelocity || policy == kNoop) {
         // rho = 1 and the prescribed velocity (see collide_store_f64)
-        if (m.drho != m.drho) atomicOr(a.flag, 1);
+        const float ux = (float)a.bc_cur[e], uy = (float)a.bc_cur[a.g.E + e];
         drho = 0.f;
-        jx = (float)a.bc_cur[e];
-        jy = (float)a.bc_cur[a.g.E + e];
+        jx = F16 ? ux * a.scale : ux;
+        jy = F16 ? uy * a.scale : uy;
         rho_out = 1.0;
-        ux_out = jx;
-        uy_out = jy;
+        ux_out = ux;
+        uy_out = uy;
     } else {
-        if (m.drho != m.drho) atomicOr(a.flag, 1);
-        drho = (float)m.drho;
-        jx = (float)m.jx;
-        jy = (float)m.jy;
-        rho_out = 1.0 + m.drho;
-        ux_out = m.jx / rho_out;
-        uy_out = m.jy / rho_out;
+        rho_out = 1.0 + md;
+        ux_out = mx / rho_out;
+        uy_out = my / rho_out;
     }
     if (a.macro) {
         int64_t q = (int64_t)j * a.g.nr + i, N = (int64_t)a.g.nr * a.g.nc;
@@ -760,7 +806,7 @@ __device__ __forceinline__ void collide_shifted(const StepArgs &a, const float *
         mq[N] = ux_out;
         mq[2 * N] = uy_out;
     }
-    const float rinv = __builtin_amdgcn_rcpf(1.f + drho);
+    const float rinv = __builtin_amdgcn_rcpf((F16 ? a.scale : 1.f) + drho);
     const float j15 = 1.5f * __builtin_fmaf(jx, jx, jy * jy);
     const float ca = a.sh_a, cb = a.sh_b;
     // c.j of the first direction of each pair: 1 (+x), 2 (+y), 5 (+x+y), 6 (-x+y)
@@ -775,8 +821,8 @@ __device__ __forceinline__ void collide_shifted(const StepArgs &a, const float *
         const float seq = __builtin_fmaf(2.f * kWh[d], sw, (2.f * kWl[d]) * sw);
         const float aeq = __builtin_fmaf(kW6h[d], c, kW6l[d] * c);
         seq_sum = k == 0 ? seq : seq_sum + seq;
-        const float ds = seq - m.s[k];
-        const float da = aeq - m.a[k];
+        const float ds = seq - ps[k];
+        const float da = aeq - pa[k];
         const float t = cb * da;
         o[d] = g[d] + __builtin_fmaf(ca, ds, t);
         o[q] = g[q] + __builtin_fmaf(ca, ds, -t);
@@ -786,14 +832,15 @@ __device__ __forceinline__ void collide_shifted(const StepArgs &a, const float *
 }
 
 // Converts the post-collision shifted populations to the storage type and
-// stores them (whole_lines: non-temporal, see store9_stream).  FP16 rounds
-// stochastically unless the context asked for round-to-nearest: 13 random
-// bits are added below the FP16 cut of the FP32 value, which is then
-// truncated (sign-magnitude, so the same for both signs).  The bits come
-// from one hash per node of its index and its new rest population, a
-// different window per direction.  Each rounding mode stores in its own
-// branch so the uniform flag stays a branch instead of both conversions
-// being computed and selected.
+// stores them (whole_lines: non-temporal, see store9_stream).  FP16 values
+// arrive scaled (collide_shifted<true>).  FP16 rounds stochastically unless
+// the context asked for round-to-nearest: 13 random bits are added below
+// the FP16 cut of the FP32 value, which is then converted toward zero
+// (sign-magnitude, so the same for both signs; two values per packed
+// conversion).  The bits come from one hash per node of its index and its
+// new rest population, a different window per direction.  Each rounding
+// mode stores in its own branch so the uniform flag stays a branch instead
+// of both conversions being computed and selected.
 template <typename T>
 __device__ __forceinline__ void encode_store(const StepArgs &a, int64_t n, const float *o, bool whole_lines)
 {
@@ -806,11 +853,18 @@ __device__ __forceinline__ void encode_store(const StepArgs &a, int64_t n, const
             h ^= h >> 13;
             h *= 0xC2B2AE35u;
             h ^= h >> 16;
+            float u[10];
 #pragma unroll
             for (int d = 0; d < 9; ++d) {
                 const uint32_t r = __builtin_amdgcn_alignbit(h, h, 3 * d) & 0x1FFFu;
-                const uint32_t b = (__float_as_uint(o[d] * a.scale) + r) & ~0x1FFFu;
-                raw[d] = (T)__uint_as_float(b);
+                u[d] = __uint_as_float(__float_as_uint(o[d]) + r);
+            }
+            u[9] = 0.f;
+#pragma unroll
+            for (int d = 0; d < 9; d += 2) {
+                const auto p = __builtin_amdgcn_cvt_pkrtz(u[d], u[d + 1]);
+                raw[d] = (T)p[0];
+                if (d + 1 < 9) raw[d + 1] = (T)p[1];
             }
             if (whole_lines) store9_stream(a, n, raw);
             else store9(a, n, raw);
@@ -818,7 +872,7 @@ __device__ __forceinline__ void encode_store(const StepArgs &a, int64_t n, const
         }
     }
 #pragma unroll
-    for (int d = 0; d < 9; ++d) raw[d] = from_g<T>(o[d], a.scale);
+    for (int d = 0; d < 9; ++d) raw[d] = (T)o[d];
     if (whole_lines) store9_stream(a, n, raw);
     else store9(a, n, raw);
 }
@@ -828,7 +882,7 @@ __device__ __forceinline__ void collide_store_shifted(const StepArgs &a, const f
                                                       int j, int64_t n)
 {
     float o[9];
-    collide_shifted(a, g, policy, e, i, j, o);
+    collide_shifted<sizeof(T) == 2>(a, g, policy, e, i, j, o);
     encode_store<T>(a, n, o, false);
 }
 
@@ -842,6 +896,11 @@ __device__ __forceinline__ void node_general_shifted(const StepArgs &a, int i, i
     float g[9];
     int64_t e = 0;
     int policy = load_g<T, GATHER>(a, i, j, x, m, edge, e, g);
+    if constexpr (sizeof(T) == 2) {
+        // the closures above work on f - w; the FP16 collision on (f - w) S
+#pragma unroll
+        for (int d = 0; d < 9; ++d) g[d] *= a.scale;
+    }
     collide_store_shifted<T>(a, g, policy, e, i, j, x.src[0]);
 }
 
@@ -884,11 +943,12 @@ __device__ __forceinline__ bool perimeter_segment(const Geometry &g, int j, int6
 // bounce-back otherwise) and only then collide and store, so up to R*9
 // loads per lane are in flight.  Keeping the general path out of this
 // kernel keeps its register allocation to what the fast path needs.
+// Callers pass row0 < ldr (segments past the column's end are skipped one
+// by one).
 template <typename T, bool GATHER, int R>
 __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0, int lane)
 {
     using V = typename ComputeOf<T>::type;
-    if (row0 >= a.g.ldr) return;
     if (!GATHER) {
 #pragma unroll 1
         for (int r = 0; r < R; ++r) {
@@ -907,8 +967,8 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
         const int64_t i0 = row0 + 64 * r;
         act[r] = false;
         own[r] = 0;
-        if (i0 < a.g.ldr) {
-            const Nbhd h = load_nbhd(a.g, j, i0);
+        if (r == 0 || i0 < a.g.ldr) {
+            const Nbhd h = load_nbhd<true>(a.g, j, i0);
             if (!perimeter_segment(a.g, j, i0, h.mid[1]) && h.mid[1] != 0) {
                 const Idx x = lane_idx(h, lane);
                 own[r] = x.src[0];
@@ -928,8 +988,7 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
                                        raw);
 #pragma unroll
                 for (int d = 0; d < 9; ++d) {
-                    if constexpr (sizeof(T) == 8) p[r][d] = raw[d];
-                    else p[r][d] = to_g<T>(raw[d], a.inv_scale);
+                    p[r][d] = (V)raw[d];  // FP16: scaled, see collide_shifted
                 }
             }
         }
@@ -947,7 +1006,7 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
         for (int r = 0; r < R; ++r) {
             if (!act[r]) continue;
             const int i = (int)row0 + 64 * r + lane;
-            collide_shifted(a, p[r], -1, 0, i, j, out[r]);
+            collide_shifted<false>(a, p[r], -1, 0, i, j, out[r]);
         }
         uint32_t b0 = (uint32_t)own[0] * 4u, b1 = (uint32_t)own[1] * 4u;
         if ((lines & 3u) == 3u) {
@@ -978,7 +1037,7 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
                 else store9(a, own[r], out);
             } else {
                 float out[9];
-                collide_shifted(a, p[r], -1, 0, i, j, out);
+                collide_shifted<sizeof(T) == 2>(a, p[r], -1, 0, i, j, out);
                 encode_store<T>(a, own[r], out, (lines >> r) & 1);
             }
         }

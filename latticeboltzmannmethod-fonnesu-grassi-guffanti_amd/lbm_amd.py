@@ -39,7 +39,7 @@ EXPORTED = [
     "lbm_get_scale", "lbm_geometry_cavity", "lbm_geometry_channel", "lbm_geometry_add_disc",
     "lbm_geometry_porous", "lbm_set_launch", "lbm_get_launch", "lbm_autotune", "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_set_launch", "lbm_batch_get_launch",
     "lbm_batch_autotune", "lbm_batch_step", "lbm_batch_reduce", "lbm_batch_info", "lbm_batch_get_stream",
-    "lbm_batch_synchronize",
+    "lbm_batch_synchronize", "lbm_copy_bandwidth",
 ]
 
 
@@ -116,6 +116,7 @@ def lib():
             "lbm_batch_info": [vp, i64p, i64p],
             "lbm_batch_get_stream": [vp, C.POINTER(vp)],
             "lbm_batch_synchronize": [vp],
+            "lbm_copy_bandwidth": [C.c_int32, C.c_int64, C.c_int32, dp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -167,6 +168,16 @@ def geometry_porous(cls, seed, rmin, rmax, target_solid):
     _check(lib().lbm_geometry_porous(nr, nc, cls.ctypes.data, seed, rmin, rmax, target_solid, C.byref(solid),
                                      C.byref(nd)))
     return solid.value, nd.value
+
+
+# ---- measurement -----------------------------------------------------------
+
+def copy_bandwidth(nbytes, reps=20, device=0):
+    """Practical HBM ceiling of a plain stream: the best rate (read + write
+    GB/s) of the library's copy kernel over nbytes (lbm_copy_bandwidth)."""
+    gbs = C.c_double(0)
+    _check(lib().lbm_copy_bandwidth(device, int(nbytes), reps, C.byref(gbs)))
+    return gbs.value
 
 
 # ---- lattice ---------------------------------------------------------------

@@ -236,3 +236,13 @@ def test_autotune_keeps_state_and_results():
             assert np.array_equal(lat.populations(), ref.populations()), (storage, seg)
             lat.close()
         ref.close()
+
+
+def test_copy_probe_measures_a_plausible_rate():
+    """The copy-kernel ceiling the benchmark reports: positive, below the
+    part's 8 TB/s nominal peak, and above a floor no working HBM stream
+    misses."""
+    say("case copy probe, 1 GiB (4x the last-level cache)")
+    gbs = L.copy_bandwidth(1 << 30, reps=10)
+    say(f"  copy kernel: {gbs:.0f} GB/s")
+    assert 1000.0 < gbs < 8000.0

@@ -104,3 +104,12 @@ def test_workload_builders():
     cav = W.cavity()
     assert np.isclose(cav.tau, 3 * 0.1 * 127 / 100 + 0.5)
     assert cav.wet_count() == 127 * 127 + 127
+
+
+def test_copy_probe_rejects_bad_sizes_before_touching_a_device():
+    """lbm_copy_bandwidth validates its arguments first: these calls make no
+    HIP call, so they run without a GPU."""
+    for nbytes, reps in ((1000, 5), (1 << 20, 0), (1 << 20, 5000)):
+        with pytest.raises(L.LbmError) as e:
+            L.copy_bandwidth(nbytes, reps)
+        assert e.value.code == 1
