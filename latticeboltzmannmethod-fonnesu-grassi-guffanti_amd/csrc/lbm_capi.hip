@@ -380,7 +380,10 @@ int lbm_set_geometry(lbm_ctx *c, const uint8_t *cls)
         }
         slots = acc;
     }
-    c->plane = std::max<int64_t>(64, ((slots + 63) / 64) * 64);
+    // At least one unused slot at the end of every plane: the step kernels'
+    // dry lanes read their own slot, which past the last wet node of the
+    // lattice is index `slots`.
+    c->plane = ((slots + 1 + 63) / 64) * 64;
     // The kernels address a population as a plane base pointer plus a
     // 32-bit byte offset spanning up to three planes.
     if ((uint64_t)3 * (uint64_t)c->plane * c->esize >= (1ull << 32)) {

@@ -934,17 +934,26 @@ __device__ __forceinline__ bool perimeter_segment(const Geometry &g, int j, int6
 }
 
 // One wave's share of a time step: R consecutive 64-row segments of
-// column j starting at row row0.  The collide-only first step (!GATHER)
-// runs every node through the general path.  Otherwise segments holding a
-// wet perimeter node are left to the perimeter kernel, and all others --
-// including those whose perimeter rows are dry walls, which the zero guard
-// bits turn into bounce-back -- first issue the loads of every segment
-// (plain streaming when the 3x3 neighbourhood is all wet, selected
-// bounce-back otherwise) and only then collide and store, so up to R*9
-// loads per lane are in flight.  Keeping the general path out of this
-// kernel keeps its register allocation to what the fast path needs.
-// Callers pass row0 < ldr (segments past the column's end are skipped one
-// by one).
+// column j starting at row row0 (callers pass row0 < ldr).  The
+// collide-only first step (!GATHER) runs every node through the general
+// path.  Otherwise segments holding a wet perimeter node are left to the
+// perimeter kernel, and all others -- including those whose perimeter rows
+// are dry walls, which the zero guard bits turn into bounce-back -- stream
+// every direction when their 3x3 neighbourhood is all wet (a scalar,
+// wave-uniform test) and otherwise select each source between the
+// neighbour and the node's own opposite slot with the wave-uniform source
+// masks.
+//
+// Loads first, for every segment and every lane, with no branch between
+// them: a load under a branch leaves its register to be merged at the
+// join, and the merge waits for the data, which serialised the segments
+// (one round trip to memory per segment instead of one per wave).  Dry
+// lanes therefore read their own slot (bounce-back address, always in
+// range: a plane holds at least one slot more than the lattice uses), and
+// segments the wave does not compute (perimeter, all dry, past the
+// column's end) read slot 0.  Collision and stores then run per segment
+// under the wet lanes' mask.  Keeping the general path out of this kernel
+// keeps its register allocation to what the fast path needs.
 template <typename T, bool GATHER, int R>
 __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0, int lane)
 {
@@ -958,40 +967,32 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
         }
         return;
     }
-    V p[R][9];
-    bool act[R];
+    T raw[R][9];
     int32_t own[R];
-    uint32_t lines = 0;    // segments whose stores are whole, aligned lines
+    uint64_t wet[R];   // the segment's wet lanes; 0 = segment not computed here
+    uint32_t lines = 0;  // segments whose stores are whole, aligned lines
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int64_t i0 = row0 + 64 * r;
-        act[r] = false;
-        own[r] = 0;
-        if (r == 0 || i0 < a.g.ldr) {
-            const Nbhd h = load_nbhd<true>(a.g, j, i0);
-            if (!perimeter_segment(a.g, j, i0, h.mid[1]) && h.mid[1] != 0) {
-                const Idx x = lane_idx(h, lane);
-                own[r] = x.src[0];
-                // A segment whose 3x3 neighbourhood is all wet (a scalar,
-                // wave-uniform test) streams every direction; otherwise the
-                // wave-uniform source masks select bounce-back sources.
-                SrcMasks sm = src_masks(h);
-                if (wave_is_bulk(h)) {
+        const bool in = r == 0 || i0 < a.g.ldr;
+        Nbhd h = load_nbhd<true>(a.g, j, in ? i0 : row0);
+        const bool go = in && !perimeter_segment(a.g, j, i0, h.mid[1]) && h.mid[1] != 0;
+        const bool bulk = go && wave_is_bulk(h);
+        if (bulk && (h.base[1] & 63) == 0) lines |= 1u << r;
+        SrcMasks sm = src_masks(h);
 #pragma unroll
-                    for (int d = 0; d < 9; ++d) sm.src[d] = ~0ull;
-                    if ((h.base[1] & 63) == 0) lines |= 1u << r;
-                }
-                act[r] = __builtin_amdgcn_inverse_ballot_w64(sm.src[0]);
-                T raw[9];
-                if (act[r])
-                    gather_interior_by(a, x, [&sm](int d) { return __builtin_amdgcn_inverse_ballot_w64(sm.src[d]); },
-                                       raw);
+        for (int d = 1; d < 9; ++d) sm.src[d] = bulk ? ~0ull : (go ? sm.src[d] & h.mid[1] : 0ull);
+        if (!go) {
 #pragma unroll
-                for (int d = 0; d < 9; ++d) {
-                    p[r][d] = (V)raw[d];  // FP16: scaled, see collide_shifted
-                }
+            for (int c = 0; c < 3; ++c) {
+                h.up[c] = h.mid[c] = h.dn[c] = 0;
+                h.base[c] = 0;
             }
         }
+        wet[r] = go ? h.mid[1] : 0ull;
+        const Idx x = lane_idx(h, lane);
+        own[r] = x.src[0];
+        gather_interior_by(a, x, [&sm](int d) { return __builtin_amdgcn_inverse_ballot_w64(sm.src[d]); }, raw[r]);
     }
     if constexpr (sizeof(T) == 4 && R == 2) {
         // FP32, two segments: collide both, then store plane by plane
@@ -1001,12 +1002,13 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
         // first part (measured: +6 % HBM writes in segment order).  With
         // more segments, or 2- and 8-byte storage, the longer live ranges
         // cost more in register spills than this saves.
+        const bool act[2] = {__builtin_amdgcn_inverse_ballot_w64(wet[0]), __builtin_amdgcn_inverse_ballot_w64(wet[1])};
         T out[R][9];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             if (!act[r]) continue;
             const int i = (int)row0 + 64 * r + lane;
-            collide_shifted<false>(a, p[r], -1, 0, i, j, out[r]);
+            collide_shifted<false>(a, raw[r], -1, 0, i, j, out[r]);
         }
         uint32_t b0 = (uint32_t)own[0] * 4u, b1 = (uint32_t)own[1] * 4u;
         if ((lines & 3u) == 3u) {
@@ -1028,16 +1030,19 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
     } else {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            if (!act[r]) continue;
+            if (wet[r] == 0 || !__builtin_amdgcn_inverse_ballot_w64(wet[r])) continue;
             const int i = (int)row0 + 64 * r + lane;
+            V p[9];
+#pragma unroll
+            for (int d = 0; d < 9; ++d) p[d] = (V)raw[r][d];  // FP16: scaled, see collide_shifted
             if constexpr (sizeof(T) == 8) {
                 double out[9];
-                collide_f64(a, p[r], -1, 0, i, j, out);
+                collide_f64(a, p, -1, 0, i, j, out);
                 if ((lines >> r) & 1) store9_stream(a, own[r], out);
                 else store9(a, own[r], out);
             } else {
                 float out[9];
-                collide_shifted<sizeof(T) == 2>(a, p[r], -1, 0, i, j, out);
+                collide_shifted<sizeof(T) == 2>(a, p, -1, 0, i, j, out);
                 encode_store<T>(a, own[r], out, (lines >> r) & 1);
             }
         }
