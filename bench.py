@@ -111,17 +111,17 @@ def build_workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def measured_traffic(wl, name, segments):
-    """HBM bytes per launch of the step kernel for this workload, storage and
-    tile shape, from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised
-    in profiles/ (calibrated against a plain copy kernel); None when that
+def measured_traffic(key, kernel="step_kernel"):
+    """HBM bytes per launch of a step kernel for a workload/storage/tile-shape
+    key, from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
+    profiles/ (calibrated against a plain copy kernel); None when that
     kernel was not profiled."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         table = json.load(f)
-    e = table.get("step_kernel", {}).get(f"{wl.name}/{name}/R{segments}")
+    e = table.get(kernel, {}).get(key)
     return None if e is None else e["read_bytes"] + e["write_bytes"]
 
 
@@ -146,7 +146,7 @@ def time_storage(wl, name, steps, warmup, ev):
     return lat, dict(storage=name, mlups=mlups, mlups_all_nodes=wl.nr * wl.nc / per_step_s / 1e6,
                      ms_per_step=ms / steps, wall_s=wall, alg_bytes_per_step=alg, achieved_gbs=achieved,
                      frac=achieved / HBM_PEAK_GBS, segments=segments,
-                     traffic_bytes_per_step=measured_traffic(wl, name, segments))
+                     traffic_bytes_per_step=measured_traffic(f"{wl.name}/{name}/R{segments}"))
 
 
 def cpu_baseline(wl, seconds_budget=20.0):
@@ -228,7 +228,9 @@ def bench_ensemble(args, ev):
     per_step_s = ms / 1e3 / args.steps
     res = dict(samples=len(wls), wet_nodes=wet, storage="f32", mlups=wet / per_step_s / 1e6,
                ms_per_step=ms / args.steps, alg_bytes_per_step=alg, achieved_gbs=alg / per_step_s / 1e9,
-               frac=alg / per_step_s / 1e9 / HBM_PEAK_GBS, segments=seg, blocks=blocks, graph_steps=32)
+               frac=alg / per_step_s / 1e9 / HBM_PEAK_GBS, segments=seg, blocks=blocks, graph_steps=32,
+               traffic_bytes_per_step=(measured_traffic(f"ensemble{len(wls)}/f32/R{seg}", "batch_kernel")
+                                       if blocks == 0 else None))
     log(f"  ensemble f32: {res['mlups']:.1f} MLUPS, {res['ms_per_step']:.4f} ms/step, {res['achieved_gbs']:.0f} GB/s "
         f"= {100 * res['frac']:.1f}% of 8 TB/s (R={seg}, "
         f"{'one wave per tile' if blocks == 0 else f'{blocks} persistent work-groups'})")

@@ -733,8 +733,9 @@ __device__ __forceinline__ void moments_shifted(const float *g, Moments &m)
 // the moments are summed exactly in FP64 (moments_shifted).  F16 = true:
 // g = (f - w) S with S the FP16 storage scale, a power of two, so every
 // FP32 operation below gives exactly S (or S^2) times what it gives on
-// f - w and the scale never needs applying; the moments are summed in
-// FP32, whose rounding is far below the storage's 11-bit resolution.
+// f - w and the scale never needs applying; the momentum is summed in
+// FP32, whose rounding is far below the storage's 11-bit resolution, and
+// the density exactly.
 //
 // Per opposite pair (d, q = opp d), with c = c_d and j = rho u:
 //   geq_d + geq_q = 2 w (drho + (4.5 (c.j)^2 - 1.5 j.j) / rho)
@@ -762,10 +763,15 @@ __device__ __forceinline__ void collide_shifted(const StepArgs &a, const float *
             ps[k] = g[d] + g[kOpp[d]];
             pa[k] = g[d] - g[kOpp[d]];
         }
-        drho = g[0] + ((ps[0] + ps[1]) + (ps[2] + ps[3]));
+        // the density exactly (FP64, rounded once): the rest equilibrium is
+        // the remainder drho - sum of the pairs, so this sum is what keeps
+        // the collision's mass balance to one rounding
+        const double dsum = (double)g[0] + ((((double)g[1] + g[3]) + ((double)g[2] + g[4])) +
+                                            (((double)g[5] + g[7]) + ((double)g[6] + g[8])));
+        drho = (float)dsum;
         jx = pa[0] + (pa[2] - pa[3]);
         jy = pa[1] + (pa[2] + pa[3]);
-        md = (double)drho * a.inv_scale;
+        md = dsum * a.inv_scale;
         mx = (double)jx * a.inv_scale;
         my = (double)jy * a.inv_scale;
     } else {
