@@ -141,3 +141,43 @@ def test_shifted_storage_on_tiny_lattices(storage, nr, nc):
     assert err <= (1e-6 if storage == L.F32_SHIFTED else 1e-3)
     ref.close()
     lo.close()
+
+
+def compact_slots(cls):
+    """Compact slot count of a node-class plane: each column starts at the
+    first index congruent to the row of its first wet node modulo 64 and
+    holds its wet nodes contiguously (the layout lbm_set_geometry builds)."""
+    acc = 0
+    for col in wet_mask(cls):
+        rows = np.nonzero(col)[0]
+        if len(rows):
+            acc += (int(rows[0]) - acc) % 64
+            acc += len(rows)
+    return acc
+
+
+def test_dry_lanes_past_the_last_wet_node():
+    """The last wet node of the lattice ends a 64-slot group and its wave has
+    dry lanes after it: those lanes read the slot one past the last one used
+    (their own, for bounce-back), which every plane keeps spare."""
+    nr, nc = 66, 10
+    cls = boxed(nr, nc)
+    cls[nc - 2, 30] = L.OBSTACLE          # shifts the last column's slots off the row numbering
+    cls[0, 1:nr - 1] = L.INLET
+    slots = compact_slots(cls)
+    last_row = int(np.nonzero(wet_mask(cls)[nc - 2])[0][-1])
+    assert slots % 64 == 0 and last_row % 64 != 63, (slots, last_row)
+    run_case("dry_tail", cls, L.BGK, L.ZOUHE_VELOCITY, L.OPEN_NOOP, 200, seed=13)
+    # the shifted formats read the same addresses
+    for storage in (L.F32_SHIFTED, L.F16_SHIFTED):
+        wl = W.Workload(name="dry_tail", node_class=cls, collision=L.BGK, tau=0.8, inlet_policy=L.ZOUHE_VELOCITY,
+                        outlet_policy=L.OPEN_NOOP, inlet=dict(ux=np.full(nr - 2, 0.02), uy=np.zeros(nr - 2)),
+                        outlet={})
+        ref, lo = wl.make_lattice(L.F64), wl.make_lattice(storage)
+        ref.step(200)
+        lo.step(200, L.STEP_CHECK)
+        err = max_abs_wet(lo.populations(), ref.populations(), cls)
+        say(f"  storage {storage}: max|dP| vs FP64 = {err:.3e}")
+        assert err <= (1e-6 if storage == L.F32_SHIFTED else 1e-3)
+        ref.close()
+        lo.close()
