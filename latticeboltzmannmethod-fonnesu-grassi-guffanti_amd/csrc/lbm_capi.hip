@@ -79,6 +79,8 @@ StepArgs make_args(const lbm_ctx *c)
     a.inv_scale = 1.f / c->scale;
     a.f16_sr = (c->desc.options & LBM_OPT_F16_ROUND_TO_NEAREST) ? 0 : 1;
     a.flag = c->d_flag;
+    a.runs = c->d_runs;
+    a.nruns = c->nruns;
     return a;
 }
 
@@ -128,6 +130,7 @@ void free_geometry(lbm_ctx *c)
     drop_graphs(c);
     (void)hipFree(c->d_wet); c->d_wet = nullptr;
     (void)hipFree(c->d_sb); c->d_sb = nullptr;
+    (void)hipFree(c->d_runs); c->d_runs = nullptr; c->nruns = 0;
     (void)hipFree(c->d_edge_kind); c->d_edge_kind = nullptr;
     (void)hipFree(c->d_frozen); c->d_frozen = nullptr;
     for (int k = 0; k < 2; ++k) {
@@ -423,6 +426,48 @@ int lbm_set_geometry(lbm_ctx *c, const uint8_t *cls)
     c->h_wet = std::move(wet);
     HIP_TRY(hipMalloc(&c->d_sb, sb.size() * sizeof(int32_t)));
     HIP_TRY(hipMemcpy(c->d_sb, sb.data(), sb.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    // Wet runs of the run kernel: in every interior column, the wet rows of
+    // the segments the perimeter kernel does not take (the first segment
+    // when row 0 is wet, the one holding row NR-1 when that row is wet),
+    // cut into runs of at most 64 rows that span at most 128 rows.  Entry:
+    // column, first row, its compact index, number of rows.
+    {
+        std::vector<int32_t> runs;
+        auto push = [&](int64_t j, int64_t i, int64_t s0, int64_t n) {
+            const int32_t e[4] = {(int32_t)j, (int32_t)i, (int32_t)s0, (int32_t)n};
+            runs.insert(runs.end(), e, e + 4);
+        };
+        const int64_t last = nr - 1;
+        for (int64_t j = 1; j + 1 < nc; ++j) {
+            const size_t w0 = (size_t)((j + 1) * c->wl + 1);
+            const std::vector<uint64_t> &hw = c->h_wet;
+            const int64_t klo = (hw[w0] & 1) ? 1 : 0;
+            const int64_t khi = ((hw[w0 + (size_t)(last / 64)] >> (last % 64)) & 1) ? last / 64 : nrw;
+            int64_t i_first = 0, s_first = 0, n = 0;
+            for (int64_t k = klo; k < khi; ++k) {
+                uint64_t m = hw[w0 + (size_t)k];
+                int64_t slot = sb[w0 + (size_t)k];
+                while (m) {
+                    const int64_t i = 64 * k + __builtin_ctzll(m);
+                    m &= m - 1;
+                    if (n == 64 || (n > 0 && i >= i_first + 128)) {
+                        push(j, i_first, s_first, n);
+                        n = 0;
+                    }
+                    if (n == 0) {
+                        i_first = i;
+                        s_first = slot;
+                    }
+                    ++n;
+                    ++slot;
+                }
+            }
+            if (n > 0) push(j, i_first, s_first, n);
+        }
+        c->nruns = (int64_t)(runs.size() / 4);
+        HIP_TRY(hipMalloc(&c->d_runs, std::max<size_t>(runs.size(), 4) * sizeof(int32_t)));
+        HIP_TRY(hipMemcpy(c->d_runs, runs.data(), runs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
     HIP_TRY(hipMalloc(&c->d_edge_kind, (size_t)c->E));
     HIP_TRY(hipMemcpy(c->d_edge_kind, edge.data(), (size_t)c->E, hipMemcpyHostToDevice));
     HIP_TRY(hipMalloc(&c->d_frozen, (size_t)(9 * c->E) * c->esize));
@@ -595,8 +640,8 @@ int lbm_step(lbm_ctx *c, int64_t n, uint32_t flags)
 int lbm_set_launch(lbm_ctx *c, int32_t segments)
 {
     if (!c) return fail(LBM_ERR_INVALID, "null context");
-    if (segments != 0 && segments != 1 && segments != 2 && segments != 4 && segments != 8)
-        return fail(LBM_ERR_INVALID, "segments must be 0, 1, 2, 4 or 8");
+    if (segments != 0 && segments != 1 && segments != 2 && segments != 4 && segments != 8 && segments != kRunLaunch)
+        return fail(LBM_ERR_INVALID, "segments must be 0, 1, 2, 4, 8 or %d (wet runs)", kRunLaunch);
     c->segments = segments;
     drop_graphs(c);
     return LBM_OK;
@@ -627,7 +672,7 @@ int lbm_autotune(lbm_ctx *c, int32_t steps, int32_t *segments, double *ms_per_st
     a.fin = c->d_F[c->cur];
     a.fout = c->d_F[1 - c->cur];
     a.macro = nullptr;
-    for (int32_t R : {1, 2, 4, 8}) {
+    for (int32_t R : {1, 2, 4, 8, kRunLaunch}) {
         if (err == hipSuccess) err = launch_step(c->storage, true, a, c->stream, R);
         if (err == hipSuccess) err = hipEventRecord(t0, c->stream);
         for (int32_t k = 0; k < steps && err == hipSuccess; ++k) err = launch_step(c->storage, true, a, c->stream, R);

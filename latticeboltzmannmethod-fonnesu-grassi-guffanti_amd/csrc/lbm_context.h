@@ -43,6 +43,8 @@ struct lbm_ctx {
     uint64_t *d_wet = nullptr;
     std::vector<uint64_t> h_wet;  // host copy of the wet words (tile tables)
     int32_t *d_sb = nullptr;
+    int32_t *d_runs = nullptr;  // wet-run table of the run kernel
+    int64_t nruns = 0;
     uint8_t *d_edge_kind = nullptr;
     void *d_frozen = nullptr;
     void *d_F[2] = {nullptr, nullptr};

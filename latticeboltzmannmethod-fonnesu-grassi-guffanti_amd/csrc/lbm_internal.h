@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "../../include/lbm_amd.h"
+
 namespace lbm {
 
 enum Storage : int32_t { kF64 = 0, kF32 = 1, kF16 = 2 };
@@ -67,7 +69,17 @@ struct StepArgs {
     int32_t f16_sr;                     // FP16 storage rounds stochastically
     double *macro;          // optional [3][NC][NR]: rho, ux, uy
     int32_t *flag;          // bit 0: NaN density seen
+    // Wet-run table of the run kernel (kRunLaunch): per run, 4 int32 --
+    // column, first row, compact index of the first row, wet rows (<= 64,
+    // all within 128 rows of the first); only rows of segments the
+    // perimeter kernel does not take.
+    const int32_t *runs;
+    int64_t nruns;
 };
+
+// Launch shape value selecting the run kernel: one wave per run of up to 64
+// consecutive wet rows of a column instead of R 64-row segments.
+constexpr int32_t kRunLaunch = LBM_LAUNCH_WET_RUNS;
 
 // Fills fin_plane / fout_plane from fin / fout.
 void set_planes(StepArgs &a);

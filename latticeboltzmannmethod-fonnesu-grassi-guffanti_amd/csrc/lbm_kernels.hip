@@ -1055,6 +1055,112 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
     }
 }
 
+// One wave's share of a gathering step in the run layout: run t of the
+// wet-run table, up to 64 consecutive wet rows of column j from row i0,
+// all within 128 rows.  Lane l takes the l-th of them, so in porous media
+// no lane idles on a solid node (a 64-row segment leaves those lanes
+// empty).  A lane's own compact index is the run's first index plus l.
+// Its row comes from the column's mask words: each wet row of the window
+// writes itself to this wave's LDS slots at its rank.  The neighbour
+// columns' words and segment bases are scalars (5 words and 3 bases per
+// column around the window); every lane selects its own from them, and
+// the neighbour indices and source masks follow as in lane_idx.  Lanes
+// past the run read their first row's slot (bounce-back addresses, in
+// range) and do not store.  Loads come before any branch (see step_wave).
+template <typename T>
+__device__ __forceinline__ void run_wave(const StepArgs &a, int64_t t, int lane, int32_t *rows)
+{
+    using V = typename ComputeOf<T>::type;
+    const LBM_CONST int32_t *tp = (const LBM_CONST int32_t *)a.runs + 4 * t;
+    const int j = __builtin_amdgcn_readfirstlane(tp[0]);
+    const int i0 = __builtin_amdgcn_readfirstlane(tp[1]);
+    const int s0 = __builtin_amdgcn_readfirstlane(tp[2]);
+    const int n = __builtin_amdgcn_readfirstlane(tp[3]);
+    const int w0 = i0 >> 6;
+    const int64_t S = a.g.ldr / 64;
+    // words w0-1 .. w0+3 of columns j-1, j, j+1 (past the column's last
+    // word: its zero guard), segment bases of words w0 .. w0+2
+    uint64_t W[3][5];
+    int32_t B[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const LBM_CONST uint64_t *col = (const LBM_CONST uint64_t *)a.g.wet + (int64_t)(j + c) * a.g.wl + 1;
+        const LBM_CONST int32_t *scol = (const LBM_CONST int32_t *)a.g.sb + (int64_t)(j + c) * a.g.wl + 1;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            const int64_t k = w0 - 1 + q;
+            W[c][q] = col[k < S ? k : S];
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int64_t k = w0 + q;
+            B[c][q] = scol[k < S ? k : S - 1];
+        }
+    }
+    // rows of the run: wet row r of the window goes to rows[rank]
+    int prefix = 0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        uint64_t m = W[1][q + 1];
+        if (q == 0) m &= ~0ull << (i0 & 63);
+        if ((m >> lane) & 1) {
+            const int r = prefix + (int)below(m);
+            if (r < n) rows[r] = 64 * (w0 + q) + lane;
+        }
+        prefix += __builtin_popcountll(m);
+    }
+    __builtin_amdgcn_wave_barrier();
+    const bool act = lane < n;
+    const int i = act ? rows[lane] : i0;
+    const int kk = (i >> 6) - w0;  // 0, 1 or 2
+    const int bp = i & 63;
+    auto pick = [kk](uint64_t x0, uint64_t x1, uint64_t x2) { return kk == 0 ? x0 : (kk == 1 ? x1 : x2); };
+    uint32_t wu[3], wm[3], wd[3];
+    int32_t rank[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const uint64_t cur = pick(W[c][1], W[c][2], W[c][3]);
+        const uint64_t up = (cur << 1) | (pick(W[c][0], W[c][1], W[c][2]) >> 63);
+        const uint64_t dn = (cur >> 1) | (pick(W[c][2], W[c][3], W[c][4]) << 63);
+        wu[c] = (uint32_t)(up >> bp) & 1u;
+        wm[c] = (uint32_t)(cur >> bp) & 1u;
+        wd[c] = (uint32_t)(dn >> bp) & 1u;
+        const int32_t base = kk == 0 ? B[c][0] : (kk == 1 ? B[c][1] : B[c][2]);
+        rank[c] = base + (int32_t)__builtin_popcountll(cur & ((1ull << bp) - 1ull));
+    }
+    const int32_t own = act ? s0 + lane : s0;
+    Idx x;
+    x.src[0] = own;
+    x.src[1] = rank[0];
+    x.src[2] = own + 1;
+    x.src[3] = rank[2];
+    x.src[4] = own - 1;
+    x.src[5] = rank[0] + (int32_t)wm[0];
+    x.src[6] = rank[2] + (int32_t)wm[2];
+    x.src[7] = rank[2] - 1;
+    x.src[8] = rank[0] - 1;
+    const uint32_t wet = act ? (wm[0] << 1) | (wd[1] << 2) | (wm[2] << 3) | (wu[1] << 4) | (wd[0] << 5) |
+                                   (wd[2] << 6) | (wu[2] << 7) | (wu[0] << 8)
+                             : 0u;
+    T raw[9];
+    gather_interior_by(a, x, [wet](int d) { return ((wet >> d) & 1u) != 0; }, raw);
+    if (!act) return;
+    V p[9];
+#pragma unroll
+    for (int d = 0; d < 9; ++d) p[d] = (V)raw[d];  // FP16: scaled, see collide_shifted
+    const bool lines = n == 64 && (s0 & 63) == 0;
+    if constexpr (sizeof(T) == 8) {
+        double out[9];
+        collide_f64(a, p, -1, 0, i, j, out);
+        if (lines) store9_stream(a, own, out);
+        else store9(a, own, out);
+    } else {
+        float out[9];
+        collide_shifted<sizeof(T) == 2>(a, p, -1, 0, i, j, out);
+        encode_store<T>(a, own, out, lines);
+    }
+}
+
 // The perimeter share of a gathering step, item w of 2 S + 2 (NC - 2)
 // (S = 64-row segments per column): items [0, S) are the segments of
 // column 0, [S, 2S) those of column NC-1, and each interior column has two
@@ -1097,6 +1203,19 @@ __global__ __launch_bounds__(kBlockRows) void step_kernel(StepArgs a)
     if (j >= a.g.nc) return;
     const int64_t row0 = (gw - (int64_t)j * a.wpc) * (64 * R);
     step_wave<T, GATHER, R>(a, j, row0, lane);
+}
+
+// The time-step kernel of one lattice in the run layout: one wave per run
+// of the wet-run table (XCD-remapped, four waves per work-group).
+template <typename T>
+__global__ __launch_bounds__(kBlockRows) void run_kernel(StepArgs a)
+{
+    __shared__ int32_t rows[kBlockRows / 64][64];
+    const int64_t b = xcd_remap(blockIdx.x, a.nblocks);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t t = b * (kBlockRows / 64) + wave;
+    if (t >= a.nruns) return;
+    run_wave<T>(a, t, threadIdx.x & 63, rows[wave]);
 }
 
 // The perimeter share of a gathering step of one lattice (perimeter_wave):
@@ -1498,8 +1617,22 @@ hipError_t launch_step_t(bool gather, StepArgs a, hipStream_t s)
 }
 
 template <typename T>
+hipError_t launch_step_runs(StepArgs a, hipStream_t s)
+{
+    hipError_t e = hipSuccess;
+    if (a.nruns > 0) e = launch(run_kernel<T>, (a.nruns + kBlockRows / 64 - 1) / (kBlockRows / 64), s, a);
+    if (e != hipSuccess) return e;
+    const int64_t items = 2 * (a.g.ldr / 64) + 2 * (a.g.nc - 2);
+    return launch(perimeter_kernel<T>, (items + kBlockRows / 64 - 1) / (kBlockRows / 64), s, a);
+}
+
+template <typename T>
 hipError_t launch_step_r(bool gather, int32_t R, const StepArgs &a, hipStream_t s)
 {
+    if (R == kRunLaunch) {
+        if (gather) return launch_step_runs<T>(a, s);
+        R = default_segments(sizeof(T) == 8 ? kF64 : kF32);
+    }
     switch (R) {
     case 1: return launch_step_t<T, 1>(gather, a, s);
     case 2: return launch_step_t<T, 2>(gather, a, s);

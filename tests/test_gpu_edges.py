@@ -49,32 +49,38 @@ def run_case(name, cls, collision, inlet_policy, outlet_policy, steps, seed=0, t
                     inlet=open_values(cls, 0, rng, inlet_policy), outlet=open_values(cls, 1, rng, outlet_policy))
     say(f"case {name}: {cls.shape[1]}x{cls.shape[0]}, {steps} steps, collision {collision}, "
         f"inlet {inlet_policy}, outlet {outlet_policy}")
-    gpu = wl.make_lattice(L.F64)
+    # the same run with row-segment waves (default shape) and wet-run waves
+    gpus = [wl.make_lattice(L.F64), wl.make_lattice(L.F64)]
+    gpus[1].set_launch(L.LAUNCH_WET_RUNS)
     orc = oracle_from(wl, threads=THREADS)
     # the prescribed values change half way (per-step update functions)
     half = steps // 2
-    gpu.step(half)
+    for g in gpus:
+        g.step(half)
     orc.step(half)
     for which, pol in ((0, inlet_policy), (1, outlet_policy)):
         vals = open_values(cls, which, rng, pol)
         if vals:
-            gpu.set_open_values(which, **vals)
+            for g in gpus:
+                g.set_open_values(which, **vals)
             orc.set_open_values(which, **vals)
-    gpu.step(steps - half, L.STEP_MACRO)
+    for g in gpus:
+        g.step(steps - half, L.STEP_MACRO)
     orc.step(steps - half)
-    pg, po = gpu.populations(), orc.populations()
+    po = orc.populations()
     assert np.all(np.isfinite(po[:, wet_mask(cls)])), f"{name}: the oracle run itself is not finite"
-    err = max_abs_wet(pg, po, cls)
-    say(f"  max|dP| = {err:.3e}")
-    assert err <= TOL, f"{name}: {err}"
-    if np.any(cls == L.OBSTACLE):
-        lg, lo = gpu.lift_drag(), orc.lift_drag()
-        assert abs(lg[0] - lo[0]) <= TOL and abs(lg[1] - lo[1]) <= TOL, (lg, lo)
-    rg, ug = gpu.macro()
-    ro, uo = orc.macro()
-    m = cls == L.FLUID
-    assert np.max(np.abs(rg[m] - ro[m])) <= TOL and np.max(np.abs(ug[:, m] - uo[:, m])) <= TOL
-    gpu.close()
+    for shape, gpu in zip(("row segments", "wet runs"), gpus):
+        err = max_abs_wet(gpu.populations(), po, cls)
+        say(f"  {shape}: max|dP| = {err:.3e}")
+        assert err <= TOL, f"{name} ({shape}): {err}"
+        if np.any(cls == L.OBSTACLE):
+            lg, lo = gpu.lift_drag(), orc.lift_drag()
+            assert abs(lg[0] - lo[0]) <= TOL and abs(lg[1] - lo[1]) <= TOL, (shape, lg, lo)
+        rg, ug = gpu.macro()
+        ro, uo = orc.macro()
+        m = cls == L.FLUID
+        assert np.max(np.abs(rg[m] - ro[m])) <= TOL and np.max(np.abs(ug[:, m] - uo[:, m])) <= TOL, shape
+        gpu.close()
     orc.close()
 
 

@@ -219,17 +219,17 @@ def test_autotune_keeps_state_and_results():
     """The tile-shape autotuner must not change the state, and every tile
     shape must give the same bits."""
     wl = W.porous(256, seed=5, nc=320)
-    say("case autotune and tile shapes, porous 256x320 FP64/FP32")
-    for storage in (L.F64, L.F32_SHIFTED):
+    say("case autotune and tile shapes (row segments, wet runs), porous 256x320 FP64/FP32/FP16")
+    for storage in (L.F64, L.F32_SHIFTED, L.F16_SHIFTED):
         ref = wl.make_lattice(storage)
         ref.step(80)
-        for seg in (1, 2, 4, 8):
+        for seg in (1, 2, 4, 8, L.LAUNCH_WET_RUNS):
             lat = wl.make_lattice(storage)
             lat.step(30)
             before = lat.populations()
             if seg == 1:
                 chosen, ms = lat.autotune(steps=2)
-                assert chosen in (1, 2, 4, 8) and ms > 0
+                assert chosen in (1, 2, 4, 8, L.LAUNCH_WET_RUNS) and ms > 0
             assert np.array_equal(before, lat.populations())
             lat.set_launch(seg)
             lat.step(50)
