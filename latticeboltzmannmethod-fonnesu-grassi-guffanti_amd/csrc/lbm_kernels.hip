@@ -1716,7 +1716,8 @@ hipError_t launch_batch_reduce(int32_t storage, bool gather, int32_t R, const Ba
     return hipGetLastError();
 }
 
-int32_t default_segments(int32_t storage) { return storage == kF64 ? 2 : 4; }
+// The shapes the autotuner picks on the 8192^2 porous and open media.
+int32_t default_segments(int32_t storage) { return storage == kF16 ? 2 : 1; }
 
 hipError_t launch_step(int32_t storage, bool gather, const StepArgs &a, hipStream_t s, int32_t R)
 {
