@@ -977,6 +977,10 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
     int32_t own[R];
     uint64_t wet[R];   // the segment's wet lanes; 0 = segment not computed here
     uint32_t lines = 0;  // segments whose stores are whole, aligned lines
+    // FP64: the wave issues its loads at raised priority, so waves about
+    // to load go ahead of waves colliding (measured +1.7 % on the 8192^2
+    // porous and open media; FP32, FP16 and the wet-run waves lose with it)
+    if constexpr (sizeof(T) == 8) __builtin_amdgcn_s_setprio(3);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int64_t i0 = row0 + 64 * r;
@@ -1000,6 +1004,7 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
         own[r] = x.src[0];
         gather_interior_by(a, x, [&sm](int d) { return __builtin_amdgcn_inverse_ballot_w64(sm.src[d]); }, raw[r]);
     }
+    if constexpr (sizeof(T) == 8) __builtin_amdgcn_s_setprio(0);
     if constexpr (sizeof(T) == 4 && R == 2) {
         // FP32, two segments: collide both, then store plane by plane
         // across them.  The cache line shared by the two segments' ranges
