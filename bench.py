@@ -29,6 +29,11 @@ The CPU baseline is the repo's CPU oracle (a restatement of the reference's
 Serial/OpenMP policy structure) timed on this host on a few steps of the
 same workload.
 
+Other workloads: open8192 (no obstacles), micro8192 (the porous geometry
+from random populations w_d (1 + 1e-3 U(-1, 1)), seed 1: the step's time
+does not depend on the flow state), cavity_trt4096, porous<n>, channel,
+cavity<n>.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload porous8192]
 """
 import argparse
@@ -95,6 +100,14 @@ def copy_ceiling(nbytes, reps=20):
 def build_workload(name):
     if name == "porous8192":
         return W.porous(8192)
+    if name.startswith("micro"):
+        # kernel micro-bench: the porous geometry with random populations
+        # w_d (1 + 1e-3 U(-1, 1)), seed 1 -- the step's bytes and time do not
+        # depend on the flow state
+        wl = W.porous(int(name[5:]))
+        wl.name = f"micro{wl.nr}x{wl.nc}"
+        wl.random_state = True
+        return wl
     if name.startswith("open"):
         # the porous set-up without obstacles (pressure-driven channel)
         wl = W.porous(int(name[4:]), solid=0.0)
@@ -125,9 +138,28 @@ def measured_traffic(key, kernel="step_kernel"):
     return None if e is None else e["read_bytes"] + e["write_bytes"]
 
 
+_RANDOM_STATE = {}
+
+
+def random_state(wl):
+    """Populations w_d (1 + 1e-3 U(-1, 1)) on every node, seed 1 (the
+    micro-bench state), in the reference layout."""
+    if wl.name not in _RANDOM_STATE:
+        rng = np.random.default_rng(1)
+        w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+        p = rng.uniform(-1e-3, 1e-3, size=(9, wl.nc, wl.nr))
+        p += 1.0
+        p *= w[:, None, None]
+        _RANDOM_STATE.clear()
+        _RANDOM_STATE[wl.name] = p
+    return _RANDOM_STATE[wl.name]
+
+
 def time_storage(wl, name, steps, warmup, ev):
     storage, options = VARIANTS[name]
     lat = wl.make_lattice(storage, options=options)
+    if getattr(wl, "random_state", False):
+        lat.upload_populations(random_state(wl))
     lat.step(warmup, L.STEP_CHECK)
     segments, _ = lat.autotune(steps=5)
     lat.step(2 * 32, L.STEP_CHECK)              # capture and warm the step graphs
@@ -307,6 +339,8 @@ def main():
         lats[s] = lat
     errors = None
     error_steps = wl.steps if args.error_steps is None else args.error_steps
+    if getattr(wl, "random_state", False):
+        error_steps = 0  # errors are stated on the physical runs, not on random states
     if error_steps > 0 and "f64" in lats and len(lats) > 1:
         t1 = time.perf_counter()
         errors = stated_errors(wl, lats["f64"], {s: lat for s, lat in lats.items() if s != "f64"}, error_steps)
