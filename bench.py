@@ -12,7 +12,7 @@ against the FP64 run, and their FP64-reduced mass difference, are stated.
 
 The ensemble line advances 512 independent porous lattices (rows and
 columns in {128, ..., 1024}, porosity in [0.5, 0.9], discs r in [3, 12])
-with one persistent launch per step, HIP-graph replay and the autotuned
+with one step-kernel launch per step, HIP-graph replay and the autotuned
 tile shape, in FP32; its per-sample Darcy permeabilities (FP64 GPU
 reductions) are checked against an FP64 ensemble.
 

@@ -226,8 +226,10 @@ int lbm_autotune(lbm_ctx *ctx, int32_t steps, int32_t *segments, double *ms_per_
  * process (Lattice.hpp:287-425).  An ensemble groups lattices that were
  * set up as usual (lbm_create, lbm_set_geometry, lbm_set_open_values,
  * lbm_init_equilibrium or lbm_upload_populations) and advances all of them
- * with ONE persistent kernel launch per time step over a table of tiles
- * (sample, column, 64*R rows).  Each sample's results are those of
+ * with one step-kernel launch per time step over a table of tiles
+ * (sample, column, 64*R rows) -- one wave per tile by default, or a
+ * persistent grid that walks the table -- plus one small launch for the
+ * samples' perimeter segments.  Each sample's results are those of
  * lbm_step on that sample alone, bit for bit.
  *
  * Rules: all samples share the device and the storage format and are in
