@@ -330,6 +330,11 @@ int lbm_set_geometry(lbm_ctx *c, const uint8_t *cls)
                 }
             }
         }
+    // The step kernels index waves and blocks in 32 bits: at most 2^30
+    // 64-row segments (the wet mask alone would take 8 GiB).
+    if ((nr + 63) / 64 * nc > (int64_t(1) << 30))
+        return fail(LBM_ERR_UNSUPPORTED, "lattice of %lld x %lld nodes exceeds 2^30 64-row segments",
+                    (long long)nr, (long long)nc);
     free_geometry(c);
     c->nr = nr;
     c->nc = nc;
