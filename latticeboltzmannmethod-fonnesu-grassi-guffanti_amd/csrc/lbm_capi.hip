@@ -81,6 +81,8 @@ StepArgs make_args(const lbm_ctx *c)
     a.flag = c->d_flag;
     a.runs = c->d_runs;
     a.nruns = c->nruns;
+    a.perim = c->d_perim;
+    a.nperim = c->nperim;
     return a;
 }
 
@@ -131,6 +133,7 @@ void free_geometry(lbm_ctx *c)
     (void)hipFree(c->d_wet); c->d_wet = nullptr;
     (void)hipFree(c->d_sb); c->d_sb = nullptr;
     (void)hipFree(c->d_runs); c->d_runs = nullptr; c->nruns = 0;
+    (void)hipFree(c->d_perim); c->d_perim = nullptr; c->nperim = 0;
     (void)hipFree(c->d_edge_kind); c->d_edge_kind = nullptr;
     (void)hipFree(c->d_frozen); c->d_frozen = nullptr;
     for (int k = 0; k < 2; ++k) {
@@ -472,6 +475,34 @@ int lbm_set_geometry(lbm_ctx *c, const uint8_t *cls)
         c->nruns = (int64_t)(runs.size() / 4);
         HIP_TRY(hipMalloc(&c->d_runs, std::max<size_t>(runs.size(), 4) * sizeof(int32_t)));
         HIP_TRY(hipMemcpy(c->d_runs, runs.data(), runs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    // Perimeter items: the 64-row segments holding a wet perimeter node,
+    // which the step kernels skip (every wet segment of columns 0 and NC-1;
+    // of the other columns the first segment when row 0 is wet and the one
+    // holding row NR-1 when that row is wet), as (column, segment).
+    {
+        std::vector<int32_t> items;
+        const std::vector<uint64_t> &hw = c->h_wet;
+        const int64_t last = nr - 1, klast = last / 64;
+        auto add = [&](int64_t j, int64_t k) {
+            items.push_back((int32_t)j);
+            items.push_back((int32_t)k);
+        };
+        for (int64_t j = 0; j < nc; ++j) {
+            const size_t w0 = (size_t)((j + 1) * c->wl + 1);
+            if (j == 0 || j == nc - 1) {
+                for (int64_t k = 0; k < nrw; ++k)
+                    if (hw[w0 + (size_t)k]) add(j, k);
+                continue;
+            }
+            const bool top = hw[w0] & 1;
+            const bool bottom = (hw[w0 + (size_t)klast] >> (last % 64)) & 1;
+            if (top) add(j, 0);
+            if (bottom && !(top && klast == 0)) add(j, klast);
+        }
+        c->nperim = (int64_t)(items.size() / 2);
+        HIP_TRY(hipMalloc(&c->d_perim, std::max<size_t>(items.size(), 2) * sizeof(int32_t)));
+        HIP_TRY(hipMemcpy(c->d_perim, items.data(), items.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     }
     HIP_TRY(hipMalloc(&c->d_edge_kind, (size_t)c->E));
     HIP_TRY(hipMemcpy(c->d_edge_kind, edge.data(), (size_t)c->E, hipMemcpyHostToDevice));

@@ -75,6 +75,11 @@ struct StepArgs {
     // perimeter kernel does not take.
     const int32_t *runs;
     int64_t nruns;
+    // Perimeter items of the perimeter kernel: per 64-row segment holding a
+    // wet perimeter node (the segments the step kernel skips), 2 int32 --
+    // column and segment index.
+    const int32_t *perim;
+    int64_t nperim;
 };
 
 // Launch shape value selecting the run kernel: one wave per run of up to 64

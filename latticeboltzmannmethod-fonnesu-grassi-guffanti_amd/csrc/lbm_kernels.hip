@@ -1172,35 +1172,6 @@ __device__ __forceinline__ void run_wave(const StepArgs &a, int64_t t, int lane,
     }
 }
 
-// The perimeter share of a gathering step, item w of 2 S + 2 (NC - 2)
-// (S = 64-row segments per column): items [0, S) are the segments of
-// column 0, [S, 2S) those of column NC-1, and each interior column has two
-// items, its first and its last segment.  Only segments holding a wet
-// perimeter node do work (the same test the step kernel skips them by).
-template <typename T>
-__device__ __forceinline__ void perimeter_wave(const StepArgs &a, int64_t w, int lane)
-{
-    const int64_t S = a.g.ldr / 64;
-    int j;
-    int64_t k;
-    if (w < 2 * S) {
-        j = w < S ? 0 : a.g.nc - 1;
-        k = w < S ? w : w - S;
-    } else {
-        const int64_t v = w - 2 * S;
-        j = 1 + (int)(v >> 1);
-        if (j >= a.g.nc - 1) return;
-        k = (v & 1) ? S - 1 : 0;
-        if ((v & 1) && S == 1) return;  // one segment: item 0 covers it
-    }
-    const int64_t i0 = 64 * k;
-    const uint64_t mid = a.g.wet[(int64_t)(j + 1) * a.g.wl + 1 + k];
-    if (!perimeter_segment(a.g, j, i0, __builtin_amdgcn_readfirstlane((uint32_t)mid) |
-                                            ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(mid >> 32)) << 32)))
-        return;
-    general_segment<T, true>(a, j, i0, lane);
-}
-
 // The time-step kernel of one lattice: wave w of the (XCD-remapped) grid
 // owns segments [w mod wpc] * R .. of column w / wpc.
 // (A multiply-high by host-computed magic numbers in place of the division
@@ -1232,16 +1203,17 @@ __global__ __launch_bounds__(kBlockRows) void run_kernel(StepArgs a)
     run_wave<T>(a, t, threadIdx.x & 63, rows[wave]);
 }
 
-// The perimeter share of a gathering step of one lattice (perimeter_wave):
-// one wave per item, 2 S + 2 (NC - 2) items.
+// The perimeter share of a gathering step of one lattice: one wave per
+// entry (column, segment) of the perimeter table, which lists exactly the
+// segments the step kernels skip (perimeter_segment), so every wave works.
 template <typename T>
 __global__ __launch_bounds__(kBlockRows) void perimeter_kernel(StepArgs a)
 {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t w = (int64_t)blockIdx.x * (kBlockRows / 64) + wave;
-    const int64_t items = 2 * (a.g.ldr / 64) + 2 * (int64_t)(a.g.nc - 2);
-    if (w >= items) return;
-    perimeter_wave<T>(a, w, threadIdx.x & 63);
+    if (w >= a.nperim) return;
+    const LBM_CONST int32_t *item = (const LBM_CONST int32_t *)a.perim + 2 * w;
+    general_segment<T, true>(a, item[0], 64 * (int64_t)item[1], threadIdx.x & 63);
 }
 
 // Entry t of a batch tile table (sample, column, first row) and the step
@@ -1618,6 +1590,13 @@ int64_t step_blocks(const StepArgs &a, int32_t R)
     return (waves + (kBlockRows / 64) - 1) / (kBlockRows / 64);
 }
 
+template <typename T>
+hipError_t launch_perimeter(const StepArgs &a, hipStream_t s)
+{
+    if (a.nperim == 0) return hipSuccess;
+    return launch(perimeter_kernel<T>, (a.nperim + kBlockRows / 64 - 1) / (kBlockRows / 64), s, a);
+}
+
 template <typename T, int R>
 hipError_t launch_step_t(bool gather, StepArgs a, hipStream_t s)
 {
@@ -1626,8 +1605,7 @@ hipError_t launch_step_t(bool gather, StepArgs a, hipStream_t s)
     if (!gather) return launch(step_kernel<T, false, R>, nb, s, a);
     hipError_t e = launch(step_kernel<T, true, R>, nb, s, a);
     if (e != hipSuccess) return e;
-    const int64_t items = 2 * (a.g.ldr / 64) + 2 * (a.g.nc - 2);
-    return launch(perimeter_kernel<T>, (items + kBlockRows / 64 - 1) / (kBlockRows / 64), s, a);
+    return launch_perimeter<T>(a, s);
 }
 
 template <typename T>
@@ -1636,8 +1614,7 @@ hipError_t launch_step_runs(StepArgs a, hipStream_t s)
     hipError_t e = hipSuccess;
     if (a.nruns > 0) e = launch(run_kernel<T>, (a.nruns + kBlockRows / 64 - 1) / (kBlockRows / 64), s, a);
     if (e != hipSuccess) return e;
-    const int64_t items = 2 * (a.g.ldr / 64) + 2 * (a.g.nc - 2);
-    return launch(perimeter_kernel<T>, (items + kBlockRows / 64 - 1) / (kBlockRows / 64), s, a);
+    return launch_perimeter<T>(a, s);
 }
 
 template <typename T>
