@@ -114,6 +114,18 @@ def test_open_boundaries_on_every_side(inlet_policy, outlet_policy):
     run_case(f"allsides{inlet_policy}{outlet_policy}", cls, L.BGK, inlet_policy, outlet_policy, 400, seed=7)
 
 
+def test_wet_top_and_bottom_rows_in_one_segment():
+    """NR <= 64 with wet nodes on row 0 and row NR-1: every interior column
+    has one 64-row segment holding both, which the perimeter kernel must take
+    exactly once."""
+    nr, nc = 40, 50
+    cls = boxed(nr, nc)
+    cls[1:nc - 1, 0] = L.INLET          # top row
+    cls[1:nc - 1, nr - 1] = L.OUTLET    # bottom row
+    L.add_disc(cls, 20, 25, 5)
+    run_case("onesegment_top_bottom", cls, L.BGK, L.ZOUHE_VELOCITY, L.ZOUHE_PRESSURE, 300, seed=11)
+
+
 def test_fluid_on_the_perimeter():
     """Fluid nodes on the domain edge (no wall): the out-of-domain slots keep
     their initial values, the same definition in oracle and kernel."""
