@@ -123,7 +123,12 @@ int lbm_set_relaxation(lbm_ctx *ctx, double tau, double tau_even, double tau_odd
  * identify_obstacle_propagation (NodeIdentification.hpp:45-242).
  * Inlet/outlet nodes must lie on the domain perimeter (the reference
  * asserts it, LatticeReader.hpp:288-300); SOLID nodes next to a wet node
- * are LBM_ERR_UNSUPPORTED. */
+ * are LBM_ERR_UNSUPPORTED.  Size limits (LBM_ERR_UNSUPPORTED, no
+ * counterpart in the reference): the kernels address populations with
+ * 32-bit byte offsets spanning three planes, so 3 * plane * element size
+ * must stay below 4 GiB (plane = wet nodes + 1 rounded up to 64; about
+ * 179M wet nodes for FP64, twice that for FP32), and wave indices are
+ * 32-bit, so at most 2^30 64-row segments (ceil(NR / 64) * NC). */
 int lbm_set_geometry(lbm_ctx *ctx, const uint8_t *node_class);
 
 /* Number of nodes per class after set_geometry: out[0..5] indexed by
