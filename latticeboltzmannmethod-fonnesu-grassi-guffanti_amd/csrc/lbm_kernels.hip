@@ -1264,17 +1264,26 @@ __global__ __launch_bounds__(kBlockRows) void batch_perimeter_kernel(BatchArgs b
 // the host (one set per buffer parity) and read in place through a
 // wave-uniform pointer, so they live in scalar registers or are reloaded
 // from the scalar cache instead of occupying vector registers.
-template <typename T, bool GATHER, int R>
+template <typename T, bool GATHER, int R, bool DIRECT>
 __global__ __launch_bounds__(kBlockRows) void batch_kernel(BatchArgs bt)
 {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (bt.direct) {
-        // one wave per tile; each XCD takes one contiguous run of the table
-        const int64_t t = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * (kBlockRows / 64) + wave;
+    if constexpr (DIRECT) {
+        // one wave per tile; each XCD takes one contiguous run of the table.
+        // The grid has ceil(ntiles / 4) work-groups (one when the table is
+        // empty), counted here from ntiles rather than read from the
+        // dispatch packet: one scalar round trip less before the tile.
+        asm volatile("" ::"s"(bt.ntiles), "s"(bt.tiles), "s"(bt.args), "s"(bt.parity));  // one round trip
+        const uint32_t nb = (uint32_t)((bt.ntiles + kBlockRows / 64 - 1) / (kBlockRows / 64));
+        const int64_t t = (int64_t)xcd_remap(blockIdx.x, nb) * (kBlockRows / 64) + wave;
         if (t >= bt.ntiles) return;
         const TileRef tr = tile_at(bt.tiles, t);
         const StepArgs a = args_at(bt, tr.s);
+        // the plane pointers are loaded with the sample's other arguments,
+        // not after its mask words arrive
+        asm volatile("" ::"s"(a.fin_plane[0]), "s"(a.fin_plane[1]), "s"(a.fin_plane[2]), "s"(a.fin_plane[5]),
+                     "s"(a.fin_plane[6]), "s"(a.g.plane));
         step_wave<T, GATHER, R>(a, tr.j, tr.row0, lane);
         return;
     }
@@ -1637,10 +1646,12 @@ hipError_t launch_batch_t(bool gather, const BatchArgs &b, int64_t blocks, hipSt
 {
     dim3 grid((unsigned)blocks), block(kBlockRows);
     if (!gather) {
-        hipLaunchKernelGGL((batch_kernel<T, false, R>), grid, block, 0, s, b);
+        if (b.direct) hipLaunchKernelGGL((batch_kernel<T, false, R, true>), grid, block, 0, s, b);
+        else hipLaunchKernelGGL((batch_kernel<T, false, R, false>), grid, block, 0, s, b);
         return hipGetLastError();
     }
-    hipLaunchKernelGGL((batch_kernel<T, true, R>), grid, block, 0, s, b);
+    if (b.direct) hipLaunchKernelGGL((batch_kernel<T, true, R, true>), grid, block, 0, s, b);
+    else hipLaunchKernelGGL((batch_kernel<T, true, R, false>), grid, block, 0, s, b);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || b.nptiles == 0) return e;
     const int64_t pb = (b.nptiles + kBlockRows / 64 - 1) / (kBlockRows / 64);
