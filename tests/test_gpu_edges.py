@@ -114,16 +114,18 @@ def test_open_boundaries_on_every_side(inlet_policy, outlet_policy):
     run_case(f"allsides{inlet_policy}{outlet_policy}", cls, L.BGK, inlet_policy, outlet_policy, 400, seed=7)
 
 
-def test_wet_top_and_bottom_rows_in_one_segment():
-    """NR <= 64 with wet nodes on row 0 and row NR-1: every interior column
-    has one 64-row segment holding both, which the perimeter kernel must take
-    exactly once."""
-    nr, nc = 40, 50
+@pytest.mark.parametrize("nr", [40, 64, 65, 128])
+def test_wet_top_and_bottom_rows(nr):
+    """Wet nodes on row 0 and row NR-1 of every interior column: with
+    NR <= 64 one 64-row segment holds both (the perimeter kernel takes it
+    once); with NR = 65 the last segment holds row NR-1 alone; with NR = 128
+    row NR-1 ends a full segment."""
+    nc = 50
     cls = boxed(nr, nc)
     cls[1:nc - 1, 0] = L.INLET          # top row
     cls[1:nc - 1, nr - 1] = L.OUTLET    # bottom row
-    L.add_disc(cls, 20, 25, 5)
-    run_case("onesegment_top_bottom", cls, L.BGK, L.ZOUHE_VELOCITY, L.ZOUHE_PRESSURE, 300, seed=11)
+    L.add_disc(cls, nr // 2, 25, min(5, nr // 2 - 3))
+    run_case(f"top_bottom{nr}", cls, L.BGK, L.ZOUHE_VELOCITY, L.ZOUHE_PRESSURE, 300, seed=11)
 
 
 def test_fluid_on_the_perimeter():
