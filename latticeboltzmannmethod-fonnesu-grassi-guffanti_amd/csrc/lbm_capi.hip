@@ -333,9 +333,10 @@ int lbm_set_geometry(lbm_ctx *c, const uint8_t *cls)
                 }
             }
         }
-    // The step kernels index waves and blocks in 32 bits: at most 2^30
-    // 64-row segments (the wet mask alone would take 8 GiB).
-    if ((nr + 63) / 64 * nc > (int64_t(1) << 30))
+    // The step kernels index waves, blocks and mask words in 32 bits: at
+    // most 2^30 64-row segments and 2^31 mask words (with the guard words
+    // and columns); the wet mask alone would take 8 GiB.
+    if ((nr + 63) / 64 * nc > (int64_t(1) << 30) || (nc + 2) * ((nr + 63) / 64 + 4) > (int64_t(1) << 31))
         return fail(LBM_ERR_UNSUPPORTED, "lattice of %lld x %lld nodes exceeds 2^30 64-row segments",
                     (long long)nr, (long long)nc);
     free_geometry(c);
@@ -343,7 +344,10 @@ int lbm_set_geometry(lbm_ctx *c, const uint8_t *cls)
     c->nc = nc;
     const int64_t nrw = (nr + 63) / 64;
     c->ldr = nrw * 64;
-    c->wl = nrw + 2;
+    // one zero guard word before each column's segments and three after
+    // them (the wet-run waves read up to two words past a column's last
+    // segment)
+    c->wl = nrw + 4;
     c->E = 2 * nc + 2 * nr;
     c->segs = (c->ldr + kBlockRows - 1) / kBlockRows;
     // host tables

@@ -155,8 +155,7 @@ __device__ __forceinline__ Nbhd load_nbhd(const Geometry &g, int j, int64_t i0)
     Nbhd h;
     // The mask and the segment bases never change while a kernel runs
     // (constant address space: scalar loads from any kernel).  Word indices
-    // are 32-bit: the mask has (NC + 2)(NR / 64 + 2) words, below 2^32 with
-    // at most 2^30 64-row segments and 2^30 columns (the geometry's caps).
+    // are 32-bit: the geometry caps the mask at 2^31 words.
     const uint32_t wl = (uint32_t)g.wl;
     const uint32_t k0 = (uint32_t)j * wl + (uint32_t)(i0 >> 6);  // column j-1, word k-1
     const LBM_CONST uint64_t *wet = (const LBM_CONST uint64_t *)g.wet;
@@ -1010,6 +1009,12 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
         own[r] = x.src[0];
         gather_interior_by(a, x, [&sm](int d) { return __builtin_amdgcn_inverse_ballot_w64(sm.src[d]); }, raw[r]);
     }
+    // FP16, one segment: the scheduler packed loaded halves between the
+    // loads and waited for them there; this keeps all nine loads ahead of
+    // any use of their data (see run_wave).
+    if constexpr (sizeof(T) == 2 && R == 1)
+        asm volatile("" : "+v"(raw[0][0]), "+v"(raw[0][1]), "+v"(raw[0][2]), "+v"(raw[0][3]), "+v"(raw[0][4]),
+                     "+v"(raw[0][5]), "+v"(raw[0][6]), "+v"(raw[0][7]), "+v"(raw[0][8]));
     if constexpr (sizeof(T) == 8) __builtin_amdgcn_s_setprio(0);
     if constexpr (sizeof(T) == 4 && R == 2) {
         // FP32, two segments: collide both, then store plane by plane
@@ -1088,26 +1093,28 @@ __device__ __forceinline__ void run_wave(const StepArgs &a, int64_t t, int lane,
     const int s0 = __builtin_amdgcn_readfirstlane(tp[2]);
     const int n = __builtin_amdgcn_readfirstlane(tp[3]);
     const int w0 = i0 >> 6;
-    const int64_t S = a.g.ldr / 64;
-    // words w0-1 .. w0+3 of columns j-1, j, j+1 (past the column's last
-    // word: its zero guard), segment bases of words w0 .. w0+2
+    // words w0-1 .. w0+3 of columns j-1, j, j+1 and the segment bases of
+    // words w0 .. w0+2.  A column keeps three zero guard words past its last
+    // segment, so these stay inside it with no clamping: constant offsets
+    // from one pointer per column, merged loads, and one wait for all of
+    // them (named together in the asm statement below).
     uint64_t W[3][5];
     int32_t B[3][3];
+    const uint32_t wl = (uint32_t)a.g.wl;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const LBM_CONST uint64_t *col = (const LBM_CONST uint64_t *)a.g.wet + (int64_t)(j + c) * a.g.wl + 1;
-        const LBM_CONST int32_t *scol = (const LBM_CONST int32_t *)a.g.sb + (int64_t)(j + c) * a.g.wl + 1;
+        const uint32_t kc = (uint32_t)(j + c) * wl + (uint32_t)w0;  // word w0-1 of the column
+        const LBM_CONST uint64_t *col = (const LBM_CONST uint64_t *)a.g.wet + kc;
+        const LBM_CONST int32_t *scol = (const LBM_CONST int32_t *)a.g.sb + kc + 1;
 #pragma unroll
-        for (int q = 0; q < 5; ++q) {
-            const int64_t k = w0 - 1 + q;
-            W[c][q] = col[k < S ? k : S];
-        }
+        for (int q = 0; q < 5; ++q) W[c][q] = col[q];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const int64_t k = w0 + q;
-            B[c][q] = scol[k < S ? k : S - 1];
-        }
+        for (int q = 0; q < 3; ++q) B[c][q] = scol[q];
     }
+    asm volatile("" ::"s"(W[0][0]), "s"(W[0][1]), "s"(W[0][2]), "s"(W[0][3]), "s"(W[0][4]), "s"(W[1][0]),
+                 "s"(W[1][1]), "s"(W[1][2]), "s"(W[1][3]), "s"(W[1][4]), "s"(W[2][0]), "s"(W[2][1]), "s"(W[2][2]),
+                 "s"(W[2][3]), "s"(W[2][4]), "s"(B[0][0]), "s"(B[0][1]), "s"(B[0][2]), "s"(B[1][0]), "s"(B[1][1]),
+                 "s"(B[1][2]), "s"(B[2][0]), "s"(B[2][1]), "s"(B[2][2]));
     // rows of the run: wet row r of the window goes to rows[rank]
     int prefix = 0;
 #pragma unroll
@@ -1155,6 +1162,12 @@ __device__ __forceinline__ void run_wave(const StepArgs &a, int64_t t, int lane,
                              : 0u;
     T raw[9];
     gather_interior_by(a, x, [wet](int d) { return ((wet >> d) & 1u) != 0; }, raw);
+    // All nine loads issue before any of their values is used: without
+    // this the scheduler interleaves the first moments with the later
+    // loads and waits for the early data in between, so a wave spends
+    // several memory round trips where one suffices.
+    asm volatile("" : "+v"(raw[0]), "+v"(raw[1]), "+v"(raw[2]), "+v"(raw[3]), "+v"(raw[4]), "+v"(raw[5]),
+                 "+v"(raw[6]), "+v"(raw[7]), "+v"(raw[8]));
     if (!act) return;
     V p[9];
 #pragma unroll
@@ -1196,6 +1209,8 @@ template <typename T>
 __global__ __launch_bounds__(kBlockRows) void run_kernel(StepArgs a)
 {
     __shared__ int32_t rows[kBlockRows / 64][64];
+    // every kernel argument the prologue needs, in one round trip
+    asm volatile("" ::"s"(a.nblocks), "s"(a.nruns), "s"(a.runs), "s"(a.g.wet), "s"(a.g.sb), "s"(a.g.wl));
     const int64_t b = xcd_remap(blockIdx.x, (uint32_t)a.nblocks);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t t = b * (kBlockRows / 64) + wave;
