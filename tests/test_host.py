@@ -113,3 +113,14 @@ def test_copy_probe_rejects_bad_sizes_before_touching_a_device():
         with pytest.raises(L.LbmError) as e:
             L.copy_bandwidth(nbytes, reps)
         assert e.value.code == 1
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU: --gpus N > 1 exits with a message before
+    any device is touched."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in r.stderr + r.stdout
