@@ -333,3 +333,149 @@ def test_poiseuille_profile():
     assert a > 0
     assert np.max(np.abs(ux - a * shape)) < 2e-3 * np.max(ux)
     assert np.max(np.abs(u[1, nc // 2, 1:-1])) < 1e-3 * np.max(ux)
+
+
+# ------------------------------------------- velocity-prescribed open nodes
+
+def _feq_textbook(rho, ux, uy):
+    """w_d rho (1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u.u), per node (arrays over nodes)."""
+    ux = np.asarray(ux, dtype=np.float64)[None]
+    uy = np.asarray(uy, dtype=np.float64)[None]
+    cu = CX[:, None] * ux + CY[:, None] * uy
+    return W[:, None] * rho * (1 + 3 * cu + 4.5 * cu ** 2 - 1.5 * (ux * ux + uy * uy))
+
+
+def _open_box(nr, nc):
+    """Velocity inlets on the top row and the left column (corner (0, 0)
+    included: a corner-typed inlet node), Zou-He pressure outlet on the
+    right column, walls elsewhere, one obstacle."""
+    cls = _box(nr, nc)
+    cls[1:-1, 0] = O.INLET     # top row (i = 0), columns 1..nc-2
+    cls[0, 1:-1] = O.INLET     # left column (j = 0), rows 1..nr-2
+    cls[0, 0] = O.INLET        # corner
+    cls[-1, 1:-1] = O.OUTLET   # right column
+    cls[6:9, 5:8] = O.OBSTACLE
+    return cls
+
+
+def _open_values(rng, n):
+    return 0.02 + 0.03 * rng.uniform(size=n), 0.01 * rng.standard_normal(n)
+
+
+@pytest.mark.parametrize("coll", [O.BGK, O.TRT])
+def test_velocity_inlet_equilibrium_uses_unit_density(coll):
+    """Q1 (Lattice.hpp:295,357; ZouHePolicy.hpp:58,71): a velocity-prescribed
+    node keeps global_rho = 1 -- update_macro touches the fluid list only and
+    the Zou-He density goes to a by-value copy -- so its collision relaxes
+    towards feq(1, U) with the prescribed U of the step, not towards the
+    equilibrium of its own moments.  Checked against an independent numpy
+    collision on every inlet node, over several steps with new values."""
+    rng = np.random.default_rng(7)
+    nr, nc = 14, 18
+    cls = _open_box(nr, nc)
+    tau, te, to = 0.83, 0.74, 1.31
+    lat = O.OracleLattice(cls, collision=coll, tau=tau, tau_even=te, tau_odd=to,
+                          inlet_policy=O.ZOUHE_VELOCITY, outlet_policy=O.ZOUHE_PRESSURE)
+    ij, _ = lat.node_list(O.LIST_INLET)
+    n = len(ij)
+    lat.set_open_values(1, rho=0.999)
+    lat.set_open_values(0, *_open_values(rng, n))
+    lat.init()
+    lat.step(5)
+    ii, jj = ij[:, 0], ij[:, 1]
+    for _ in range(4):
+        ux, uy = _open_values(rng, n)
+        lat.set_open_values(0, ux=ux, uy=uy)
+        p = lat.populations()[:, jj, ii]
+        lat.step(1)
+        a = lat.after_collision()[:, jj, ii]
+        e = _feq_textbook(1.0, ux, uy)
+        if coll == O.BGK:
+            want = (1 - 1 / tau) * p + (1 / tau) * e
+        else:
+            sym = 0.5 * ((p + p[OPP]) - (e + e[OPP]))
+            asym = 0.5 * ((p - p[OPP]) - (e - e[OPP]))
+            want = p - sym / te - asym / to
+        assert np.max(np.abs(a - want)) < 2e-16 * 8
+        # the moments of these nodes are not (1, U): the check above would
+        # fail with the equilibrium of the moments
+        r_mom = p.sum(axis=0)
+        assert np.min(np.abs(r_mom - 1.0)) > 1e-9
+        rho, u = lat.macro()
+        assert np.all(rho[jj, ii] == 1.0)
+        assert np.all(u[0][jj, ii] == ux) and np.all(u[1][jj, ii] == uy)
+
+
+def test_zouhe_closes_each_step_with_that_steps_values():
+    """Q3 (Lattice.hpp:351,405-410; InletOutletInitializer.hpp:90-129): the
+    values set before step i (update_nodes at the top of the loop body)
+    close the Zou-He boundary at the end of step i.  After the step every
+    TOP and LEFT inlet node holds density rho_zh -- the Zou-He mass balance
+    of its known populations -- and momentum rho_zh U(i); it does not match
+    the values of step i-1 or i+1.  The corner-typed inlet (0, 0) is left
+    alone (ZouHePolicy.hpp:119-120): its out-of-domain slots keep w_d."""
+    rng = np.random.default_rng(11)
+    nr, nc = 14, 18
+    cls = _open_box(nr, nc)
+    lat = O.OracleLattice(cls, collision=O.TRT, tau_even=0.7, tau_odd=1.4,
+                          inlet_policy=O.ZOUHE_VELOCITY, outlet_policy=O.ZOUHE_PRESSURE)
+    ij, tag = lat.node_list(O.LIST_INLET)
+    n = len(ij)
+    ii, jj = ij[:, 0], ij[:, 1]
+    top, left = tag == O.BT_TOP, tag == O.BT_LEFT
+    corner = (ii == 0) & (jj == 0)
+    assert top.sum() == nc - 2 and left.sum() == nr - 2 and corner.sum() == 1
+    lat.set_open_values(1, rho=1.0)
+    U = [_open_values(rng, n) for _ in range(8)]
+    lat.set_open_values(0, ux=U[0][0], uy=U[0][1])
+    lat.init()
+    w = O.feq(1.0, 0.0, 0.0)
+    for s in range(7):
+        lat.set_open_values(0, ux=U[s][0], uy=U[s][1])
+        lat.step(1)
+        p = lat.populations()[:, jj, ii]
+        rho = p.sum(axis=0)
+        jx, jy = (CX[:, None] * p).sum(axis=0), (CY[:, None] * p).sum(axis=0)
+        sel = top | left
+        # closed-form mass balance from the known populations
+        known_top = p[0] + p[1] + p[3] + 2 * (p[2] + p[5] + p[6])
+        known_left = p[0] + p[2] + p[4] + 2 * (p[3] + p[7] + p[6])
+        rho_zh = np.where(top, known_top / (1 + U[s][1]), known_left / (1 - U[s][0]))
+        assert np.max(np.abs(rho[sel] - rho_zh[sel])) < 1e-14
+        assert np.max(np.abs(jx[sel] - rho[sel] * U[s][0][sel])) < 1e-14
+        assert np.max(np.abs(jy[sel] - rho[sel] * U[s][1][sel])) < 1e-14
+        for other in (s - 1, s + 1):
+            if other >= 0:
+                assert np.min(np.abs(jx[sel] - rho[sel] * U[other][0][sel])) > 1e-8
+        # slots of (0, 0) whose source (0, 0) - c_d lies outside the domain:
+        # push offsets with +1 row (d = 4, 7, 8) or +1 column (d = 1, 5, 8)
+        for d in (1, 4, 5, 7, 8):
+            assert p[d][corner][0] == w[d]
+
+
+def test_reduce_matches_population_sums():
+    """orc_reduce: out[0] = mass of the wet nodes, out[1], out[2] = momentum
+    of the fluid nodes, out[3] = fluid mass, out[4] = fluid count, against
+    numpy sums over populations() on a lattice with inlet, outlet and
+    obstacle nodes (so fluid != wet)."""
+    rng = np.random.default_rng(3)
+    nr, nc = 14, 18
+    cls = _open_box(nr, nc)
+    lat = O.OracleLattice(cls, collision=O.BGK, tau=0.9, inlet_policy=O.ZOUHE_VELOCITY,
+                          outlet_policy=O.ZOUHE_PRESSURE)
+    ij, _ = lat.node_list(O.LIST_INLET)
+    lat.set_open_values(0, *_open_values(rng, len(ij)))
+    lat.set_open_values(1, rho=0.998)
+    lat.init()
+    lat.step(30)
+    p = lat.populations()
+    fl = cls == O.FLUID
+    wet = wet_mask(cls)
+    out = lat.reduce()
+    pf = p[:, fl]
+    assert np.isclose(out[0], p[:, wet].sum(), rtol=1e-14, atol=0)
+    assert np.isclose(out[1], (CX[:, None] * pf).sum(), rtol=0, atol=1e-14)
+    assert np.isclose(out[2], (CY[:, None] * pf).sum(), rtol=0, atol=1e-14)
+    assert np.isclose(out[3], pf.sum(), rtol=1e-14, atol=0)
+    assert out[4] == fl.sum()
+    assert abs(out[1]) > 1e-3  # the flow carries momentum: the check has teeth
