@@ -36,7 +36,9 @@ typedef enum lbm_status {
     LBM_ERR_NAN = 3,         /* a NaN density was produced (reference: LLALBM_DEBUG check, Lattice.hpp:412-422) */
     LBM_ERR_STATE = 4,       /* call out of order (e.g. step before geometry) */
     LBM_ERR_UNSUPPORTED = 5, /* valid in the reference, not handled here (documented cases) */
-    LBM_ERR_NOMEM = 6
+    LBM_ERR_NOMEM = 6,
+    LBM_ERR_RANGE = 7        /* FP16 storage: a scaled population reached the FP16 limit (65504); the storage
+                                scale is too large for the flow (no reference counterpart) */
 } lbm_status;
 
 /* Node classes = the reference's InputNodeType (llalbm/src/utils/aliases.hpp:41-48). */
@@ -68,7 +70,7 @@ typedef enum lbm_storage {
 
 typedef enum lbm_step_flags {
     LBM_STEP_MACRO = 1u,    /* the last step of the call also stores rho/u (reference global_rho/global_u) */
-    LBM_STEP_CHECK = 2u,    /* synchronise after the call and report LBM_ERR_NAN */
+    LBM_STEP_CHECK = 2u,    /* synchronise after the call and report LBM_ERR_NAN (or LBM_ERR_RANGE) */
     LBM_STEP_NO_GRAPH = 4u  /* launch every step eagerly instead of replaying a captured HIP graph */
 } lbm_step_flags;
 
@@ -82,7 +84,8 @@ typedef struct lbm_desc {
     double tau_odd;        /* TRT tau- (TRTCollision.hpp:70) */
     int32_t inlet_policy;  /* lbm_open_policy of the inlet list  (LatticeConfiguration inlet_policy_t) */
     int32_t outlet_policy; /* lbm_open_policy of the outlet list (LatticeConfiguration outlet_policy_t) */
-    double f16_scale;      /* FP16 storage scale (power of two); 0 = chosen from the boundary values */
+    double f16_scale;      /* FP16 storage scale (a power of two in [1, 2^24]); 0 = chosen from the prescribed
+                              values of the first step (see lbm_init_equilibrium) */
     int32_t device;        /* HIP device ordinal */
     int32_t options;       /* lbm_options bits (0 = none) */
 } lbm_desc;
@@ -149,7 +152,11 @@ int lbm_open_nodes(lbm_ctx *ctx, int which, int64_t *ij, int64_t *count);
 int lbm_set_open_values(lbm_ctx *ctx, int which, const double *ux, const double *uy, const double *rho);
 
 /* Start of Lattice::perform_lbm (Lattice.hpp:294-328): rho = 1, u = 0 and
- * populations = feq(1, 0) on the wet nodes; the step counter restarts at 0. */
+ * populations = feq(1, 0) on the wet nodes; the step counter restarts at 0.
+ * FP16 storage with f16_scale = 0 picks its scale from the prescribed
+ * values here, and again from any values set after this call and before
+ * the first step (the state is then exactly zero in every scale), so the
+ * values of step 0 may be set before or after this call. */
 int lbm_init_equilibrium(lbm_ctx *ctx);
 
 /* Replaces the population state (NR*NC*9, reference layout).  Only wet
@@ -183,14 +190,47 @@ int lbm_get_time(lbm_ctx *ctx, int64_t *t);
  * nodes, rho = 1, u = 0 elsewhere. */
 int lbm_download_macro(lbm_ctx *ctx, double *rho, double *u);
 
+/* Asynchronous lbm_download_macro for output on save steps
+ * (Lattice.hpp:370-374 writes u_x/u_y every save_step-th step;
+ * LatticeWriter.hpp:61-93).  Queues the copies of the fields of the last
+ * LBM_STEP_MACRO step into rho and u (either may be NULL) on a copy stream
+ * of the context and returns at once; *ticket names the copy for
+ * lbm_wait.  The context keeps two device copies of the fields, so the
+ * copy overlaps the steps that follow; a third LBM_STEP_MACRO step waits
+ * for the copy of the first.  rho and u must stay valid until the ticket
+ * completes; with pageable host memory HIP may stage the copy
+ * synchronously, page-locked memory (lbm_host_alloc) keeps it
+ * asynchronous. */
+int lbm_download_macro_async(lbm_ctx *ctx, double *rho, double *u, int64_t *ticket);
+
+/* Blocks until the work named by a ticket of this context has completed
+ * (tickets older than the last 64 are complete already). */
+int lbm_wait(lbm_ctx *ctx, int64_t ticket);
+
+/* Page-locked host memory for asynchronous downloads (hipHostMalloc);
+ * lbm_host_free releases it.  No reference counterpart. */
+int lbm_host_alloc(int64_t bytes, void **ptr);
+int lbm_host_free(void *ptr);
+
 /* FlowAnalyzer<2>::compute_flow_properties (FlowAnalyzer.hpp:348-451) on
  * the post-collision populations of the last step, over n points given as
  * (i, j) pairs with their 9-bit direction masks (NULL, NULL = every wet
  * obstacle node, in list order).  lift_drag[0] = total lift,
  * lift_drag[1] = total drag, reduced in FP64 on the GPU.  per_point
- * (optional, 2*n) receives (lift, drag) of each point. */
+ * (optional, 2*n) receives (lift, drag) of each point.  A direction bit
+ * must name a wet neighbour (fluid, inlet or outlet) inside the lattice,
+ * as the reference's direction masks do (NodeIdentification.hpp:137-211);
+ * other masks are LBM_ERR_INVALID. */
 int lbm_lift_drag(lbm_ctx *ctx, const int64_t *ij, const int32_t *masks, int64_t n, double lift_drag[2],
                   double *per_point);
+
+/* lbm_lift_drag without waiting: the points are copied before the call
+ * returns, the results arrive in lift_drag / per_point (which must stay
+ * valid; page-locked memory keeps the copy asynchronous) when *ticket
+ * completes (lbm_wait).  Used on analysis steps so the steps after them
+ * queue behind the reduction instead of waiting for it. */
+int lbm_lift_drag_async(lbm_ctx *ctx, const int64_t *ij, const int32_t *masks, int64_t n, double *lift_drag,
+                        double *per_point, int64_t *ticket);
 
 /* FP64 reductions over the current populations: out[0] = mass of the wet
  * nodes, out[1] = x momentum and out[2] = y momentum of the fluid nodes,

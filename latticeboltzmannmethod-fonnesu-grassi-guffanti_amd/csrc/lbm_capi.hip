@@ -142,11 +142,17 @@ void free_geometry(lbm_ctx *c)
         if (c->h_bc_pin[k]) (void)hipHostFree(c->h_bc_pin[k]);
         c->h_bc_pin[k] = nullptr;
     }
-    (void)hipFree(c->d_macro); c->d_macro = nullptr;
+    for (int k = 0; k < 2; ++k) {
+        (void)hipFree(c->d_macro[k]);
+        c->d_macro[k] = nullptr;
+    }
+    c->macro_slot = 0;
     (void)hipFree(c->d_partials); c->d_partials = nullptr; c->partials_cap = 0;
     (void)hipFree(c->d_pts); c->d_pts = nullptr;
     (void)hipFree(c->d_masks); c->d_masks = nullptr;
     (void)hipFree(c->d_pp); c->d_pp = nullptr; c->pts_cap = 0;
+    c->h_pts.clear();
+    c->h_masks.clear();
     (void)hipFree(c->d_staging); c->d_staging = nullptr;
     c->has_geom = false;
     c->state = kNone;
@@ -187,7 +193,12 @@ int push_bc(lbm_ctx *c)
 
 // Storage scale of the FP16 path: a power of two that puts the largest
 // expected |f - w| (from the prescribed velocities and densities) near
-// 2^11, far from both the FP16 overflow (65504) and the subnormal range.
+// 2^11.  The estimate is at least 1/16, so the scale is at most 2^15 and
+// values up to |f - w| = 1.99 stay below the FP16 limit (65504) whatever
+// the boundary values: interior flows faster than the boundaries, and
+// inlets that ramp up from rest, fit.  The scale does not change the
+// relative resolution of normal FP16 numbers (2^-11); at 2^15 only
+// |f - w| below 2e-9 falls into the subnormal range.
 float choose_scale(const lbm_ctx *c)
 {
     if (c->storage != kF16) return 1.f;
@@ -199,9 +210,9 @@ float choose_scale(const lbm_ctx *c)
             dmax = std::max(dmax, std::fabs(c->h_bc[2 * c->E + e] - 1.0));
         }
     double gmax = (4.0 / 9.0) * (dmax + 3.0 * umax + 6.0 * umax * umax);
-    gmax = std::max(gmax, 1e-4);
+    gmax = std::max(gmax, 1.0 / 16.0);
     double s = std::exp2(std::floor(std::log2(2048.0 / gmax)));
-    s = std::min(std::max(s, 1.0), 16777216.0);
+    s = std::min(std::max(s, 1.0), 32768.0);
     return (float)s;
 }
 
@@ -220,6 +231,113 @@ int64_t edge_slot_host(int64_t i, int64_t j, int64_t nr, int64_t nc)
 
 const int kOffI[9] = {0, 0, -1, 0, 1, -1, -1, 1, 1};
 const int kOffJ[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+
+bool wet_at(const lbm_ctx *c, int64_t i, int64_t j)
+{
+    return (c->h_wet[(size_t)((j + 1) * c->wl + 1 + i / 64)] >> (i % 64)) & 1;
+}
+
+// The copy of the rho/u fields the next writer (an LBM_STEP_MACRO step or
+// init) fills: the one the last writer did not fill, once its last
+// asynchronous download has read it.
+int claim_macro(lbm_ctx *c, double **out)
+{
+    const int k = 1 - c->macro_slot;
+    HIP_TRY(hipStreamWaitEvent(c->stream, c->macro_free[k], 0));
+    c->macro_slot = k;
+    *out = c->d_macro[k];
+    return LBM_OK;
+}
+
+// Records the completion event of a new ticket on stream s.  The event
+// slot of the ticket kTickets back is reused once that ticket completed.
+int new_ticket(lbm_ctx *c, hipStream_t s, int64_t *ticket)
+{
+    const int64_t t = c->tick_seq + 1;
+    hipEvent_t &ev = c->tick_ev[t % lbm_ctx::kTickets];
+    if (!ev) HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    else HIP_TRY(hipEventSynchronize(ev));
+    HIP_TRY(hipEventRecord(ev, s));
+    c->tick_seq = t;
+    if (ticket) *ticket = t;
+    return LBM_OK;
+}
+
+// FP16 storage with a scale chosen from the prescribed values: after init
+// and before the first step the state is exactly zero in every scale, so
+// the scale follows values set in between.
+void rescale_if_fresh(lbm_ctx *c)
+{
+    if (c->storage != kF16 || c->desc.f16_scale > 0 || !c->fresh_init) return;
+    const float s = choose_scale(c);
+    if (s != c->scale) {
+        c->scale = s;
+        drop_graphs(c);
+    }
+}
+
+// Validates a lift/drag point set against the lattice (each direction bit
+// must name a wet node inside the lattice) and makes it the one held on
+// the device.  *np receives the point count.
+int stage_points(lbm_ctx *c, const int64_t *ij, const int32_t *masks, int64_t n, int64_t *np)
+{
+    if ((ij == nullptr) != (masks == nullptr)) return fail(LBM_ERR_INVALID, "ij and masks go together");
+    const int64_t *hij = ij ? ij : c->obst_ij.data();
+    const int32_t *hm = ij ? masks : c->obst_mask.data();
+    if (!ij) n = (int64_t)c->obst_mask.size();
+    if (n < 0) return fail(LBM_ERR_INVALID, "negative point count");
+    for (int64_t q = 0; q < n; ++q) {
+        int64_t i = hij[2 * q], j = hij[2 * q + 1];
+        int32_t m = hm[q];
+        if (i < 0 || i >= c->nr || j < 0 || j >= c->nc) return fail(LBM_ERR_INVALID, "point %lld outside the lattice", (long long)q);
+        if (m & ~0x1FE) return fail(LBM_ERR_INVALID, "point %lld: mask bits outside directions 1..8", (long long)q);
+        for (int d = 1; d < 9; ++d)
+            if ((m >> d) & 1) {
+                int64_t ii = i + kOffI[d], jj = j + kOffJ[d];
+                if (ii < 0 || ii >= c->nr || jj < 0 || jj >= c->nc)
+                    return fail(LBM_ERR_INVALID, "point %lld: direction %d leaves the lattice", (long long)q, d);
+                if (!wet_at(c, ii, jj))
+                    return fail(LBM_ERR_INVALID, "point %lld: direction %d names a node that is not fluid, inlet or outlet",
+                                (long long)q, d);
+            }
+    }
+    *np = n;
+    if (n == 0) return LBM_OK;
+    const bool same = (int64_t)c->h_masks.size() == n && std::equal(hm, hm + n, c->h_masks.begin()) &&
+                      std::equal(hij, hij + 2 * n, c->h_pts.begin());
+    if (same) return LBM_OK;
+    // kernels of earlier (asynchronous) calls may still read the old set
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (n > c->pts_cap) {
+        (void)hipFree(c->d_pts); (void)hipFree(c->d_masks); (void)hipFree(c->d_pp);
+        c->d_pts = nullptr; c->d_masks = nullptr; c->d_pp = nullptr; c->pts_cap = 0;
+        HIP_TRY(hipMalloc(&c->d_pts, (size_t)(2 * n) * sizeof(int64_t)));
+        HIP_TRY(hipMalloc(&c->d_masks, (size_t)n * sizeof(int32_t)));
+        HIP_TRY(hipMalloc(&c->d_pp, (size_t)(2 * n) * sizeof(double)));
+        c->pts_cap = n;
+    }
+    c->h_pts.clear();
+    c->h_masks.clear();
+    HIP_TRY(hipMemcpy(c->d_pts, hij, (size_t)(2 * n) * sizeof(int64_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_masks, hm, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice));
+    c->h_pts.assign(hij, hij + 2 * n);
+    c->h_masks.assign(hm, hm + n);
+    return LBM_OK;
+}
+
+// Queues the lift/drag reduction of the staged points: per-point values in
+// d_pp (when wanted), totals in d_out[0..1].
+int queue_lift_drag(lbm_ctx *c, int64_t n, bool per_point)
+{
+    int64_t nb = (n + kBlockRows - 1) / kBlockRows;
+    int r = ensure_partials(c, 2 * nb);
+    if (r) return r;
+    StepArgs a = make_args(c);
+    HIP_TRY(launch_lift_drag(c->storage, a, c->d_F[c->cur], c->d_pts, c->d_masks, n, per_point ? c->d_pp : nullptr,
+                             c->d_partials, &nb, c->stream));
+    HIP_TRY(launch_reduce_partials(c->d_partials, nb, 2, c->d_out, c->stream));
+    return LBM_OK;
+}
 
 }  // namespace
 
@@ -244,6 +362,13 @@ int lbm_create(const lbm_desc *desc, lbm_ctx **out)
     if (desc->collision == LBM_BGK && !(desc->tau > 0)) return fail(LBM_ERR_INVALID, "tau must be > 0");
     if (desc->collision == LBM_TRT && !(desc->tau_even > 0 && desc->tau_odd > 0))
         return fail(LBM_ERR_INVALID, "tau_even and tau_odd must be > 0");
+    if (desc->f16_scale != 0.0) {
+        // the FP16 collision relies on S being an exact power of two
+        int ex = 0;
+        const double m = std::isfinite(desc->f16_scale) ? std::frexp(desc->f16_scale, &ex) : 0.0;
+        if (!(desc->f16_scale >= 1.0 && desc->f16_scale <= 16777216.0 && m == 0.5))
+            return fail(LBM_ERR_INVALID, "f16_scale must be 0 or a power of two in [1, 2^24]");
+    }
     int ndev = 0;
     HIP_TRY(hipGetDeviceCount(&ndev));
     if (desc->device < 0 || desc->device >= ndev) return fail(LBM_ERR_INVALID, "no HIP device %d", desc->device);
@@ -257,12 +382,17 @@ int lbm_create(const lbm_desc *desc, lbm_ctx **out)
         delete c;
         return fail(LBM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
     }
-    for (int k = 0; k < 2; ++k) {
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->copy_go, hipEventDisableTiming);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) {
         e = hipEventCreateWithFlags(&c->bc_ev[k], hipEventDisableTiming);
-        if (e != hipSuccess) {
-            lbm_destroy(c);
-            return fail(LBM_ERR_HIP, "hipEventCreate: %s", hipGetErrorString(e));
-        }
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->macro_free[k], hipEventDisableTiming);
+        // both field copies start free
+        if (e == hipSuccess) e = hipEventRecord(c->macro_free[k], c->copy_stream);
+    }
+    if (e != hipSuccess) {
+        lbm_destroy(c);
+        return fail(LBM_ERR_HIP, "stream/event creation: %s", hipGetErrorString(e));
     }
     e = hipMalloc(&c->d_flag, 64);
     if (e == hipSuccess) e = hipMalloc(&c->d_out, 8 * sizeof(double));
@@ -281,11 +411,18 @@ int lbm_destroy(lbm_ctx *c)
     if (c->batch) return fail(LBM_ERR_STATE, "context belongs to an ensemble (destroy the lbm_batch first)");
     (void)hipSetDevice(c->desc.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
     free_geometry(c);
     (void)hipFree(c->d_flag);
     (void)hipFree(c->d_out);
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < 2; ++k) {
         if (c->bc_ev[k]) (void)hipEventDestroy(c->bc_ev[k]);
+        if (c->macro_free[k]) (void)hipEventDestroy(c->macro_free[k]);
+    }
+    for (hipEvent_t ev : c->tick_ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (c->copy_go) (void)hipEventDestroy(c->copy_go);
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return LBM_OK;
@@ -312,6 +449,7 @@ int lbm_set_geometry(lbm_ctx *c, const uint8_t *cls)
     c->version++;
     HIP_TRY(hipSetDevice(c->desc.device));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipStreamSynchronize(c->copy_stream));
     const int64_t nr = c->desc.n_rows, nc = c->desc.n_cols;
     // validation
     for (int64_t q = 0; q < nr * nc; ++q)
@@ -520,12 +658,14 @@ int lbm_set_geometry(lbm_ctx *c, const uint8_t *cls)
         HIP_TRY(hipMemcpy(c->d_bc[k], c->h_bc.data(), (size_t)(3 * c->E) * sizeof(double), hipMemcpyHostToDevice));
     }
     c->bc_cur = c->bc_prev = 0;
-    HIP_TRY(hipMalloc(&c->d_macro, (size_t)(3 * nr * nc) * sizeof(double)));
+    for (int k = 0; k < 2; ++k) HIP_TRY(hipMalloc(&c->d_macro[k], (size_t)(3 * nr * nc) * sizeof(double)));
+    c->macro_slot = 0;
     c->has_geom = true;
     c->state = kNone;
     c->t = 0;
     c->cur = 0;
     c->macro_valid = false;
+    c->fresh_init = false;
     return LBM_OK;
 }
 
@@ -548,19 +688,24 @@ int lbm_open_nodes(lbm_ctx *c, int which, int64_t *ij, int64_t *count)
 
 int lbm_set_open_values(lbm_ctx *c, int which, const double *ux, const double *uy, const double *rho)
 {
-    if (c) c->version++;
     if (!c || (which != 0 && which != 1)) return fail(LBM_ERR_INVALID, "bad argument");
     if (!c->has_geom) return fail(LBM_ERR_STATE, "no geometry");
-    HIP_TRY(hipSetDevice(c->desc.device));
     const auto &es = c->open_e[which];
+    // all values are checked before any is taken: a failed call changes nothing
+    for (size_t q = 0; q < es.size(); ++q) {
+        if ((ux && !std::isfinite(ux[q])) || (uy && !std::isfinite(uy[q])))
+            return fail(LBM_ERR_INVALID, "prescribed velocity of node %zu is not finite", q);
+        if (rho && !(rho[q] > 0 && std::isfinite(rho[q])))
+            return fail(LBM_ERR_INVALID, "prescribed density of node %zu must be finite and > 0", q);
+    }
+    HIP_TRY(hipSetDevice(c->desc.device));
+    c->version++;
     for (size_t q = 0; q < es.size(); ++q) {
         if (ux) c->h_bc[es[q]] = ux[q];
         if (uy) c->h_bc[c->E + es[q]] = uy[q];
-        if (rho) {
-            if (!(rho[q] > 0)) return fail(LBM_ERR_INVALID, "prescribed density must be > 0");
-            c->h_bc[2 * c->E + es[q]] = rho[q];
-        }
+        if (rho) c->h_bc[2 * c->E + es[q]] = rho[q];
     }
+    rescale_if_fresh(c);
     return push_bc(c);
 }
 
@@ -572,12 +717,16 @@ int lbm_init_equilibrium(lbm_ctx *c)
     HIP_TRY(hipSetDevice(c->desc.device));
     c->scale = choose_scale(c);
     drop_graphs(c);
+    double *macro = nullptr;
+    int r = claim_macro(c, &macro);
+    if (r) return r;
     StepArgs a = make_args(c);
-    HIP_TRY(launch_init_eq(c->storage, a, c->d_F[c->cur], c->d_frozen, c->d_macro, c->stream));
+    HIP_TRY(launch_init_eq(c->storage, a, c->d_F[c->cur], c->d_frozen, macro, c->stream));
     HIP_TRY(hipMemsetAsync(c->d_flag, 0, 64, c->stream));
     c->state = kP;
     c->t = 0;
     c->macro_valid = true;
+    c->fresh_init = true;
     return LBM_OK;
 }
 
@@ -597,6 +746,7 @@ int lbm_upload_populations(lbm_ctx *c, const double *P)
     HIP_TRY(launch_upload(c->storage, a, c->d_staging, c->d_F[c->cur], c->d_frozen, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->state = kP;
+    c->fresh_init = false;
     return LBM_OK;
 }
 
@@ -659,8 +809,13 @@ int lbm_step(lbm_ctx *c, int64_t n, uint32_t flags)
         StepArgs a = make_args(c);
         a.fin = c->d_F[c->cur];
         a.fout = c->d_F[1 - c->cur];
-        a.macro = (macro && s == n - 1) ? c->d_macro : nullptr;
+        a.macro = nullptr;
+        if (macro && s == n - 1) {
+            int r = claim_macro(c, &a.macro);
+            if (r) return r;
+        }
         HIP_TRY(launch_step(c->storage, c->state == kFstar, a, c->stream, c->segments));
+        c->fresh_init = false;
         c->cur = 1 - c->cur;
         c->state = kFstar;
         c->bc_prev = c->bc_cur;
@@ -672,7 +827,10 @@ int lbm_step(lbm_ctx *c, int64_t n, uint32_t flags)
         int32_t flag = 0;
         HIP_TRY(hipMemcpyAsync(&flag, c->d_flag, sizeof flag, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
-        if (flag) return fail(LBM_ERR_NAN, "NaN density at or before step %lld", (long long)c->t);
+        if (flag & 1) return fail(LBM_ERR_NAN, "NaN density at or before step %lld", (long long)c->t);
+        if (flag & 2)
+            return fail(LBM_ERR_RANGE, "FP16 storage: a scaled population reached 65504 at or before step %lld "
+                        "(storage scale %g is too large for this flow)", (long long)c->t, (double)c->scale);
     }
     return LBM_OK;
 }
@@ -748,9 +906,51 @@ int lbm_download_macro(lbm_ctx *c, double *rho, double *u)
     if (!c->macro_valid) return fail(LBM_ERR_STATE, "no macro fields (step with LBM_STEP_MACRO)");
     HIP_TRY(hipSetDevice(c->desc.device));
     size_t N = (size_t)(c->nr * c->nc);
-    if (rho) HIP_TRY(hipMemcpyAsync(rho, c->d_macro, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    if (u) HIP_TRY(hipMemcpyAsync(u, c->d_macro + N, 2 * N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    const double *m = c->d_macro[c->macro_slot];
+    if (rho) HIP_TRY(hipMemcpyAsync(rho, m, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (u) HIP_TRY(hipMemcpyAsync(u, m + N, 2 * N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_download_macro_async(lbm_ctx *c, double *rho, double *u, int64_t *ticket)
+{
+    if (!c || !ticket) return fail(LBM_ERR_INVALID, "null argument");
+    if (!c->macro_valid) return fail(LBM_ERR_STATE, "no macro fields (step with LBM_STEP_MACRO)");
+    HIP_TRY(hipSetDevice(c->desc.device));
+    const size_t N = (size_t)(c->nr * c->nc);
+    const int k = c->macro_slot;
+    const double *m = c->d_macro[k];
+    HIP_TRY(hipEventRecord(c->copy_go, c->stream));
+    HIP_TRY(hipStreamWaitEvent(c->copy_stream, c->copy_go, 0));
+    if (rho) HIP_TRY(hipMemcpyAsync(rho, m, N * sizeof(double), hipMemcpyDeviceToHost, c->copy_stream));
+    if (u) HIP_TRY(hipMemcpyAsync(u, m + N, 2 * N * sizeof(double), hipMemcpyDeviceToHost, c->copy_stream));
+    HIP_TRY(hipEventRecord(c->macro_free[k], c->copy_stream));
+    return new_ticket(c, c->copy_stream, ticket);
+}
+
+int lbm_wait(lbm_ctx *c, int64_t ticket)
+{
+    if (!c) return fail(LBM_ERR_INVALID, "null context");
+    if (ticket <= 0 || ticket > c->tick_seq) return fail(LBM_ERR_INVALID, "unknown ticket %lld", (long long)ticket);
+    if (ticket <= c->tick_seq - lbm_ctx::kTickets) return LBM_OK;
+    HIP_TRY(hipSetDevice(c->desc.device));
+    HIP_TRY(hipEventSynchronize(c->tick_ev[ticket % lbm_ctx::kTickets]));
+    return LBM_OK;
+}
+
+int lbm_host_alloc(int64_t bytes, void **ptr)
+{
+    if (!ptr || bytes < 0) return fail(LBM_ERR_INVALID, "bad argument");
+    *ptr = nullptr;
+    if (bytes == 0) return LBM_OK;
+    HIP_TRY(hipHostMalloc(ptr, (size_t)bytes, hipHostMallocDefault));
+    return LBM_OK;
+}
+
+int lbm_host_free(void *ptr)
+{
+    if (ptr) HIP_TRY(hipHostFree(ptr));
     return LBM_OK;
 }
 
@@ -759,49 +959,40 @@ int lbm_lift_drag(lbm_ctx *c, const int64_t *ij, const int32_t *masks, int64_t n
 {
     if (!c || !lift_drag) return fail(LBM_ERR_INVALID, "null argument");
     if (c->state != kFstar) return fail(LBM_ERR_STATE, "no step completed since the last init/upload");
-    if ((ij == nullptr) != (masks == nullptr)) return fail(LBM_ERR_INVALID, "ij and masks go together");
     HIP_TRY(hipSetDevice(c->desc.device));
-    const int64_t *hij = ij ? ij : c->obst_ij.data();
-    const int32_t *hm = ij ? masks : c->obst_mask.data();
-    if (!ij) n = (int64_t)c->obst_mask.size();
-    if (n < 0) return fail(LBM_ERR_INVALID, "negative point count");
-    for (int64_t q = 0; q < n; ++q) {
-        int64_t i = hij[2 * q], j = hij[2 * q + 1];
-        int32_t m = hm[q];
-        if (i < 0 || i >= c->nr || j < 0 || j >= c->nc) return fail(LBM_ERR_INVALID, "point %lld outside the lattice", (long long)q);
-        for (int d = 1; d < 9; ++d)
-            if ((m >> d) & 1) {
-                int64_t ii = i + kOffI[d], jj = j + kOffJ[d];
-                if (ii < 0 || ii >= c->nr || jj < 0 || jj >= c->nc)
-                    return fail(LBM_ERR_INVALID, "point %lld: direction %d leaves the lattice", (long long)q, d);
-            }
-    }
+    int r = stage_points(c, ij, masks, n, &n);
+    if (r) return r;
     if (n == 0) {
         lift_drag[0] = lift_drag[1] = 0.0;
         return LBM_OK;
     }
-    if (n > c->pts_cap) {
-        (void)hipFree(c->d_pts); (void)hipFree(c->d_masks); (void)hipFree(c->d_pp);
-        c->d_pts = nullptr; c->d_masks = nullptr; c->d_pp = nullptr; c->pts_cap = 0;
-        HIP_TRY(hipMalloc(&c->d_pts, (size_t)(2 * n) * sizeof(int64_t)));
-        HIP_TRY(hipMalloc(&c->d_masks, (size_t)n * sizeof(int32_t)));
-        HIP_TRY(hipMalloc(&c->d_pp, (size_t)(2 * n) * sizeof(double)));
-        c->pts_cap = n;
-    }
-    HIP_TRY(hipMemcpyAsync(c->d_pts, hij, (size_t)(2 * n) * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->d_masks, hm, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-    int64_t nb = (n + kBlockRows - 1) / kBlockRows;
-    int r = ensure_partials(c, 2 * nb);
+    r = queue_lift_drag(c, n, per_point != nullptr);
     if (r) return r;
-    StepArgs a = make_args(c);
-    HIP_TRY(launch_lift_drag(c->storage, a, c->d_F[c->cur], c->d_pts, c->d_masks, n, per_point ? c->d_pp : nullptr,
-                             c->d_partials, &nb, c->stream));
-    HIP_TRY(launch_reduce_partials(c->d_partials, nb, 2, c->d_out, c->stream));
     HIP_TRY(hipMemcpyAsync(lift_drag, c->d_out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     if (per_point)
         HIP_TRY(hipMemcpyAsync(per_point, c->d_pp, (size_t)(2 * n) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return LBM_OK;
+}
+
+int lbm_lift_drag_async(lbm_ctx *c, const int64_t *ij, const int32_t *masks, int64_t n, double *lift_drag,
+                        double *per_point, int64_t *ticket)
+{
+    if (!c || !lift_drag || !ticket) return fail(LBM_ERR_INVALID, "null argument");
+    if (c->state != kFstar) return fail(LBM_ERR_STATE, "no step completed since the last init/upload");
+    HIP_TRY(hipSetDevice(c->desc.device));
+    int r = stage_points(c, ij, masks, n, &n);
+    if (r) return r;
+    if (n == 0) {
+        lift_drag[0] = lift_drag[1] = 0.0;
+        return new_ticket(c, c->stream, ticket);
+    }
+    r = queue_lift_drag(c, n, per_point != nullptr);
+    if (r) return r;
+    HIP_TRY(hipMemcpyAsync(lift_drag, c->d_out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (per_point)
+        HIP_TRY(hipMemcpyAsync(per_point, c->d_pp, (size_t)(2 * n) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    return new_ticket(c, c->stream, ticket);
 }
 
 int lbm_reduce(lbm_ctx *c, double out[5])

@@ -58,13 +58,28 @@ struct lbm_ctx {
     hipEvent_t bc_ev[2] = {nullptr, nullptr};
     int bc_cur = 0, bc_prev = 0;
     std::vector<double> h_bc;  // [3][E] latest values
-    double *d_macro = nullptr;
+    // rho/u fields of LBM_STEP_MACRO steps: two device copies, written in
+    // turn, so an asynchronous download of one (on copy_stream) overlaps the
+    // steps that write the other; macro_free[k] is recorded on copy_stream
+    // after the last download of copy k, and a step writing copy k waits
+    // for it.
+    double *d_macro[2] = {nullptr, nullptr};
+    int macro_slot = 0;  // the copy the last LBM_STEP_MACRO step (or init) wrote
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t macro_free[2] = {nullptr, nullptr};
+    hipEvent_t copy_go = nullptr;  // fields complete on the main stream: the copy stream may start
+    // completion events of asynchronous downloads, by ticket modulo kTickets
+    static constexpr int kTickets = 64;
+    hipEvent_t tick_ev[kTickets] = {};
+    int64_t tick_seq = 0;
     int32_t *d_flag = nullptr;
     double *d_partials = nullptr;
     int64_t partials_cap = 0;
     double *d_out = nullptr;  // 8 doubles
     int64_t *d_pts = nullptr;
     int32_t *d_masks = nullptr;
+    std::vector<int64_t> h_pts;   // the point set held in d_pts / d_masks
+    std::vector<int32_t> h_masks;
     double *d_pp = nullptr;
     int64_t pts_cap = 0;
     double *d_staging = nullptr;
@@ -78,6 +93,10 @@ struct lbm_ctx {
     int state = lbm::kNone;
     int64_t t = 0;
     float scale = 1.f;
+    // set by lbm_init_equilibrium, cleared by the first step or an upload:
+    // the state is then exactly zero in shifted storage, so an FP16 scale
+    // chosen from the prescribed values may still change
+    bool fresh_init = false;
     bool macro_valid = false;
     // bumped by every call that changes what a step reads (geometry,
     // populations, prescribed values, relaxation) or steps the lattice

@@ -857,6 +857,15 @@ __device__ __forceinline__ void encode_store(const StepArgs &a, int64_t n, const
 {
     T raw[9];
     if constexpr (sizeof(T) == 2) {
+        // Range guard: a scaled value at or beyond the largest FP16 number
+        // (65504) would be clamped by the conversion toward zero or become
+        // infinite to nearest; bit 1 of the flag reports it (the storage
+        // scale is too large for the flow).  NaNs pass fmaxf unnoticed:
+        // the density check above reports them.
+        const float mx = fmaxf(fmaxf(fmaxf(fabsf(o[0]), fabsf(o[1])), fmaxf(fabsf(o[2]), fabsf(o[3]))),
+                               fmaxf(fmaxf(fabsf(o[4]), fabsf(o[5])),
+                                     fmaxf(fmaxf(fabsf(o[6]), fabsf(o[7])), fabsf(o[8]))));
+        if (mx >= 65504.f) atomicOr(a.flag, 2);
         if (a.f16_sr) {
             uint32_t h = (uint32_t)n * 0x9E3779B9u ^ __float_as_uint(o[0]);
             h ^= h >> 16;

@@ -105,13 +105,8 @@ public:
     /// populations of the context's last step (lbm_lift_drag).
     std::pair<double, double> compute_on_device(lbm_ctx *ctx)
     {
-        std::vector<int64_t> ij(2 * considered_points.size());
-        std::vector<int32_t> masks(considered_points.size());
-        for (std::size_t q = 0; q < considered_points.size(); ++q) {
-            ij[2 * q] = considered_points[q].coords[0];
-            ij[2 * q + 1] = considered_points[q].coords[1];
-            masks[q] = (int32_t)considered_points[q].directions.to_ulong();
-        }
+        collect();
+        stage_points();
         double ld[2] = {0.0, 0.0};
         std::vector<double> per(2 * considered_points.size());
         if (!considered_points.empty()) {
@@ -122,13 +117,104 @@ public:
         return record(ld[0], ld[1], per);
     }
 
+    /// compute_on_device without waiting (lbm_lift_drag_async): the
+    /// reduction queues behind the step on the GPU and its result lands in
+    /// page-locked memory.  collect() records the results in call order;
+    /// at most two are outstanding (a third call records the first).
+    void compute_on_device_async(lbm_ctx *ctx)
+    {
+        Pending &p = pending[next];
+        if (p.active) finish(p);
+        p.seq = seq++;
+        p.active = true;
+        p.ctx = ctx;
+        p.empty = considered_points.empty();
+        p.n = considered_points.size();
+        next ^= 1;
+        if (p.empty) return;
+        stage_points();
+        const std::size_t n = considered_points.size();
+        p.buf.reserve(2 + 2 * n);
+        int r = lbm_lift_drag_async(ctx, ij.data(), masks.data(), (int64_t)n, p.buf.data(),
+                                    should_save ? p.buf.data() + 2 : nullptr, &p.ticket);
+        if (r != LBM_OK) {
+            p.active = false;
+            throw std::runtime_error(std::string("lbm_lift_drag_async: ") + lbm_last_error());
+        }
+    }
+
+    /// Records every outstanding asynchronous result, oldest first.
+    void collect()
+    {
+        Pending &a = pending[0], &b = pending[1];
+        Pending *first = &a, *second = &b;
+        if (b.active && (!a.active || b.seq < a.seq)) std::swap(first, second);
+        if (first->active) finish(*first);
+        if (second->active) finish(*second);
+    }
+
 private:
+    /// Page-locked doubles for asynchronous results (lbm_host_alloc).
+    struct PinnedDoubles {
+        double *ptr = nullptr;
+        std::size_t cap = 0;
+        PinnedDoubles() = default;
+        PinnedDoubles(const PinnedDoubles &) = delete;
+        PinnedDoubles &operator=(const PinnedDoubles &) = delete;
+        ~PinnedDoubles() { lbm_host_free(ptr); }
+        void reserve(std::size_t n)
+        {
+            if (n <= cap) return;
+            lbm_host_free(ptr);
+            ptr = nullptr;
+            cap = 0;
+            void *p = nullptr;
+            if (lbm_host_alloc((int64_t)(n * sizeof(double)), &p) != LBM_OK)
+                throw std::runtime_error(std::string("lbm_host_alloc: ") + lbm_last_error());
+            ptr = (double *)p;
+            cap = n;
+        }
+        double *data() { return ptr; }
+    };
+    struct Pending {
+        bool active = false, empty = false;
+        int64_t ticket = 0, seq = 0;
+        std::size_t n = 0;
+        lbm_ctx *ctx = nullptr;
+        PinnedDoubles buf;
+    };
+
+    void stage_points()
+    {
+        ij.resize(2 * considered_points.size());
+        masks.resize(considered_points.size());
+        for (std::size_t q = 0; q < considered_points.size(); ++q) {
+            ij[2 * q] = considered_points[q].coords[0];
+            ij[2 * q + 1] = considered_points[q].coords[1];
+            masks[q] = (int32_t)considered_points[q].directions.to_ulong();
+        }
+    }
+
+    void finish(Pending &p)
+    {
+        p.active = false;
+        if (p.empty) {
+            record(0.0, 0.0, {});
+            return;
+        }
+        if (lbm_wait(p.ctx, p.ticket) != LBM_OK) throw std::runtime_error(std::string("lbm_wait: ") + lbm_last_error());
+        const double *b = p.buf.data();
+        std::vector<double> per;
+        if (should_save) per.assign(b + 2, b + 2 + 2 * p.n);
+        record(b[0], b[1], per);
+    }
+
     std::pair<double, double> record(double lift, double drag, const std::vector<double> &per)
     {
         if (should_save) {
             std::ofstream o("./flow_analysis/contribution_" + std::to_string(save_counter) + ".csv");
             o << "i,j,lift,drag" << std::endl;
-            for (std::size_t q = 0; q < considered_points.size(); ++q)
+            for (std::size_t q = 0; q < considered_points.size() && 2 * q + 1 < per.size(); ++q)
                 o << considered_points[q].coords[0] << "," << considered_points[q].coords[1] << "," << per[2 * q]
                   << "," << per[2 * q + 1] << std::endl;
             save_counter++;
@@ -138,6 +224,11 @@ private:
     }
 
     std::vector<ObstaclePoint<2>> considered_points;
+    std::vector<int64_t> ij;
+    std::vector<int32_t> masks;
+    Pending pending[2];
+    int next = 0;
+    int64_t seq = 0;
     std::size_t iterations_between_save = 10;
     bool should_save = true;
     int save_counter = 0;

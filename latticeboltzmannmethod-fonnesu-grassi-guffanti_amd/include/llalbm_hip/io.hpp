@@ -114,21 +114,29 @@ inline void create_output_directory()
             std::filesystem::remove(entry.path());
 }
 
-/// LatticeWriter.hpp:61-93: u_x and u_y as text rows.
-template <std::size_t dim>
-void write_lattice_file(const HostTensor<double, dim + 1> &global_u, std::size_t iteration_count)
+/// LatticeWriter.hpp:61-93: u_x and u_y as text rows, from a velocity
+/// array in the ColMajor (NR, NC, 2) layout: (i, j, k) at i + NR*(j + NC*k).
+inline void write_velocity_files(const double *u, Index nr, Index nc, std::size_t iteration_count)
 {
     std::ofstream ux(output_dir_path + "/output-" + std::to_string(iteration_count) + "u_x.txt");
     std::ofstream uy(output_dir_path + "/output-" + std::to_string(iteration_count) + "u_y.txt");
     if (!ux.is_open() || !uy.is_open()) throw std::runtime_error("output file could not be opened");
-    for (Index i = 0; i < global_u.dimension(0); ++i) {
-        for (Index j = 0; j < global_u.dimension(1); ++j) {
-            ux << global_u(i, j, 0) << " ";
-            uy << global_u(i, j, 1) << " ";
+    const double *vx = u, *vy = u + nr * nc;
+    for (Index i = 0; i < nr; ++i) {
+        for (Index j = 0; j < nc; ++j) {
+            ux << vx[i + nr * j] << " ";
+            uy << vy[i + nr * j] << " ";
         }
         ux << std::endl;
         uy << std::endl;
     }
+}
+
+/// LatticeWriter.hpp:61-93 on a velocity tensor.
+template <std::size_t dim>
+void write_lattice_file(const HostTensor<double, dim + 1> &global_u, std::size_t iteration_count)
+{
+    write_velocity_files(global_u.data(), global_u.dimension(0), global_u.dimension(1), iteration_count);
 }
 
 } // namespace llalbm::util::writer

@@ -97,61 +97,85 @@ public:
             ctx = nullptr;
         }
         build_lists();
+        host_valid = false;
+        ran = false;
+        // Lattice.hpp:196-200: the initializer keeps copies of the lists
+        ParallelizationPolicy::attach_nodes(inlet_nodes_coord, outlet_nodes_coord);
     }
 
     /// Lattice.hpp:288-438.  time / time_step steps; every save_step-th
     /// step writes results/output-<k>u_{x,y}.txt when should_save; an
     /// attached FlowAnalyzer runs at i > 0 && i % k == 0 on the
     /// post-collision populations.  Steps with no event and unchanged
-    /// inlet/outlet values are launched together.
+    /// inlet/outlet values are launched together.  Save and analysis steps
+    /// do not stall the GPU: their fields and reductions are downloaded
+    /// asynchronously into page-locked double buffers, and a save is
+    /// written to text while the GPU runs the steps queued after it.
     void perform_lbm(const double time, const double time_step = 1, const std::size_t save_step = 1,
                      const bool should_save = true)
     {
         if (!(time_step > 0) || !(time > 0) || !(time > time_step))
             throw std::invalid_argument("simulation time and time step must be positive, time > time step");
+        if (should_save && save_step == 0) throw std::invalid_argument("save_step must be positive");
         ensure_context();
         if (should_save) util::writer::create_output_directory();
         if (flow_analyzer) flow_analyzer->initialize();
         const std::size_t n_steps = (std::size_t)(time / time_step);
         std::size_t saved_file = 0;
-        lbm_check(lbm_init_equilibrium(ctx), "lbm_init_equilibrium");
         std::vector<double> vals[2][3], last[2][3];
-        auto t0 = std::chrono::high_resolution_clock::now();
-        std::int64_t pending = 0;
-        auto flush = [&]() {
-            if (pending) lbm_check(lbm_step(ctx, pending, step_flags()), "lbm_step");
-            pending = 0;
-        };
-        for (std::size_t i = 0; i < n_steps; ++i) {
-            // update_nodes (Lattice.hpp:351): values of step i
+        // update_nodes (Lattice.hpp:351) of step i; the values of step 0 go
+        // to the context before the initialisation, which picks the FP16
+        // storage scale from them
+        auto push_values = [&](std::size_t i, bool first, auto &&before_change) {
             for (int w = 0; w < 2; ++w) {
                 const auto &nodes = w == 0 ? inlet_nodes_coord : outlet_nodes_coord;
                 if (nodes.empty()) continue;
                 ParallelizationPolicy::open_values((double)i, w, nodes, vals[w][0], vals[w][1], vals[w][2]);
-                if (vals[w][0] != last[w][0] || vals[w][1] != last[w][1] || vals[w][2] != last[w][2] || i == 0) {
-                    flush();
+                if (first || vals[w][0] != last[w][0] || vals[w][1] != last[w][1] || vals[w][2] != last[w][2]) {
+                    before_change();
                     lbm_check(lbm_set_open_values(ctx, w, vals[w][0].data(), vals[w][1].data(), vals[w][2].data()),
                               "lbm_set_open_values");
                     for (int k = 0; k < 3; ++k) last[w][k] = vals[w][k];
                 }
             }
+        };
+        if (n_steps > 0) push_values(0, true, [] {});
+        lbm_check(lbm_init_equilibrium(ctx), "lbm_init_equilibrium");
+        saves.reset(lattice_dimensions[0], lattice_dimensions[1]);
+        auto t0 = std::chrono::high_resolution_clock::now();
+        std::int64_t pending = 0;
+        // Queues the steps without events; the host then writes what has
+        // arrived from earlier save and analysis steps while they run.
+        auto flush = [&]() {
+            if (pending) lbm_check(lbm_step(ctx, pending, step_flags()), "lbm_step");
+            pending = 0;
+            saves.write_all(ctx);
+            if (flow_analyzer) flow_analyzer->collect();
+        };
+        for (std::size_t i = 0; i < n_steps; ++i) {
+            if (i > 0) push_values(i, false, flush);
             const bool save = should_save && i % save_step == 0;
             const bool analyse = flow_analyzer && i > 0 && i % flow_analyzer->get_iterations_between_save() == 0;
-            if (!save && !analyse) {
+            // the last step stores rho/u too: get_global_rho/u return them
+            const bool final_step = i + 1 == n_steps;
+            if (!save && !analyse && !final_step) {
                 ++pending;
                 continue;
             }
-            flush();
-            lbm_check(lbm_step(ctx, 1, step_flags() | (save ? LBM_STEP_MACRO : 0u)), "lbm_step");
-            if (save) {
-                download_macro();
-                util::writer::write_lattice_file<2>(global_u, saved_file++);
-            }
-            if (analyse) flow_analyzer->compute_on_device(ctx);
+            if (pending) lbm_check(lbm_step(ctx, pending, step_flags()), "lbm_step");
+            pending = 0;
+            lbm_check(lbm_step(ctx, 1, step_flags() | ((save || final_step) ? LBM_STEP_MACRO : 0u)), "lbm_step");
+            if (save) saves.queue(ctx, saved_file++);
+            if (analyse) flow_analyzer->compute_on_device_async(ctx);
+            // the GPU now runs this step (and those before it): write the
+            // save before this one meanwhile
+            saves.write_older(ctx);
         }
         flush();
         lbm_check(lbm_step(ctx, 0, LBM_STEP_CHECK), "lbm_step");
         lbm_check(lbm_synchronize(ctx), "lbm_synchronize");
+        saves.write_all(ctx);
+        if (flow_analyzer) flow_analyzer->collect();
         auto t1 = std::chrono::high_resolution_clock::now();
         last_elapsed = std::chrono::duration<double>(t1 - t0).count();
 #ifdef TIMER
@@ -159,6 +183,7 @@ public:
 #endif
         if (flow_analyzer) flow_analyzer->save_global_results();
         host_valid = false;
+        ran = n_steps > 0;
     }
 
     /// Lattice.hpp:492: the populations after the completed steps.
@@ -170,7 +195,10 @@ public:
         return populations;
     }
 
-    /// Lattice.hpp:499 / 506: rho and u of the last step (LBM_STEP_MACRO).
+    /// Lattice.hpp:499 / 506: rho and u of the last step of the last run
+    /// (before any run: rho = 1, u = 0).  The tensors are the lattice's
+    /// own: writes through the returned reference (update_nodes) persist
+    /// until the next run.
     HostTensor<double, 3> &get_global_u()
     {
         if (!host_valid) download_macro();
@@ -224,6 +252,8 @@ public:
             lbm_destroy(ctx);
             ctx = nullptr;
         }
+        host_valid = false;
+        ran = false;
     }
 
     double elapsed_seconds() const { return last_elapsed; }
@@ -265,12 +295,93 @@ private:
 
     void download_macro()
     {
-        ensure_context();
         global_rho = HostTensor<double, 2>(lattice_dimensions[0], lattice_dimensions[1]);
         global_u = HostTensor<double, 3>(lattice_dimensions[0], lattice_dimensions[1], (Index)2);
-        lbm_check(lbm_download_macro(ctx, global_rho.data(), global_u.data()), "lbm_download_macro");
+        if (ran) {
+            ensure_context();
+            lbm_check(lbm_download_macro(ctx, global_rho.data(), global_u.data()), "lbm_download_macro");
+        } else {
+            // Lattice.hpp:295-296 before the first step
+            global_rho.setConstant(1.0);
+            global_u.setConstant(0.0);
+        }
         host_valid = true;
     }
+
+    /// Page-locked double buffer of the u fields of save steps: a save
+    /// queues the download of its step's fields (lbm_download_macro_async)
+    /// and is written to text (LatticeWriter.hpp:61-93) by write_all, or
+    /// when its buffer is needed again.
+    class SaveQueue
+    {
+    public:
+        SaveQueue() = default;
+        SaveQueue(const SaveQueue &) = delete;
+        SaveQueue &operator=(const SaveQueue &) = delete;
+        ~SaveQueue() { release(); }
+
+        void reset(Index nr_, Index nc_)
+        {
+            for (auto &sl : slot) sl.active = false;
+            if (nr_ != nr || nc_ != nc) release();
+            nr = nr_;
+            nc = nc_;
+        }
+        void queue(lbm_ctx *ctx, std::size_t file_index)
+        {
+            Slot &sl = slot[next];
+            if (sl.active) write(ctx, sl);
+            if (!sl.u) {
+                void *p = nullptr;
+                lbm_check(lbm_host_alloc((int64_t)(2 * nr * nc * sizeof(double)), &p), "lbm_host_alloc");
+                sl.u = (double *)p;
+            }
+            lbm_check(lbm_download_macro_async(ctx, nullptr, sl.u, &sl.ticket), "lbm_download_macro_async");
+            sl.file = file_index;
+            sl.seq = seq++;
+            sl.active = true;
+            next ^= 1;
+        }
+        /// Writes every queued save but the newest.
+        void write_older(lbm_ctx *ctx)
+        {
+            if (slot[0].active && slot[1].active) write(ctx, slot[0].seq < slot[1].seq ? slot[0] : slot[1]);
+        }
+        /// Writes every queued save, oldest first.
+        void write_all(lbm_ctx *ctx)
+        {
+            Slot *a = &slot[0], *b = &slot[1];
+            if (b->active && (!a->active || b->seq < a->seq)) std::swap(a, b);
+            if (a->active) write(ctx, *a);
+            if (b->active) write(ctx, *b);
+        }
+
+    private:
+        struct Slot {
+            double *u = nullptr;
+            int64_t ticket = 0, seq = 0;
+            std::size_t file = 0;
+            bool active = false;
+        };
+        void write(lbm_ctx *ctx, Slot &sl)
+        {
+            sl.active = false;
+            lbm_check(lbm_wait(ctx, sl.ticket), "lbm_wait");
+            util::writer::write_velocity_files(sl.u, nr, nc, sl.file);
+        }
+        void release()
+        {
+            for (auto &sl : slot) {
+                lbm_host_free(sl.u);
+                sl.u = nullptr;
+                sl.active = false;
+            }
+        }
+        Slot slot[2];
+        int next = 0;
+        int64_t seq = 0;
+        Index nr = 0, nc = 0;
+    };
 
     /// The reference's node lists: row-major order, btypes by position
     /// (NodeIdentification.hpp:45-110), obstacle direction masks and
@@ -337,6 +448,8 @@ private:
     HostTensor<double, 3> populations, global_u;
     HostTensor<double, 2> global_rho;
     bool host_valid = false;
+    bool ran = false;  // a run has completed since the geometry was set
+    SaveQueue saves;
     int32_t storage = LBM_STORE_F64;
     lbm_ctx *ctx = nullptr;
     std::shared_ptr<analysis::FlowAnalyzer<2>> flow_analyzer;

@@ -18,7 +18,10 @@
 #include <concepts>
 #include <functional>
 #include <iostream>
+#include <stdexcept>
+#include <string>
 #include <type_traits>
+#include <vector>
 
 #include "../../../include/lbm_amd.h"
 #include "aliases.hpp"
@@ -189,15 +192,41 @@ class HIPVelocityInitializer<2> : public InitializationPolicyTag, public HIPTag
 {
 public:
     using Fn = std::function<double(double, BoundaryPoint<2>)>;
+    static inline std::vector<BoundaryPoint<2>> inlet_nodes{};
+    static inline std::vector<BoundaryPoint<2>> outlet_nodes{};
     static inline std::array<Fn, 2> inlet_update_function{};
     static inline std::array<Fn, 2> outlet_update_function{};
     static inline Fn inlet_density_function{};
     static inline Fn outlet_density_function{};
 
+    /// InletOutletInitializer.hpp:60-67: copies of the lattice's lists.
+    static void attach_nodes(std::vector<BoundaryPoint<2>> &inlet_nodes_, std::vector<BoundaryPoint<2>> &outlet_nodes_)
+    {
+        inlet_nodes = inlet_nodes_;
+        outlet_nodes = outlet_nodes_;
+    }
+
     static void attach_update_functions(std::array<Fn, 2> &inlet_fns, std::array<Fn, 2> &outlet_fns)
     {
         inlet_update_function = inlet_fns;
         outlet_update_function = outlet_fns;
+    }
+
+    /// InletOutletInitializer.hpp:90-129: the velocity tensor takes the
+    /// update functions' values at time t on the attached inlet and outlet
+    /// nodes.  The density tensor is not written (the reference receives
+    /// it by value).
+    static void update_nodes(const double &t, util::HostTensor<double, 3> &velocity_tensor,
+                             const util::HostTensor<double, 2> &density_tensor)
+    {
+        (void)density_tensor;
+        for (int w = 0; w < 2; ++w) {
+            const auto &nodes = w == 0 ? inlet_nodes : outlet_nodes;
+            const auto &fns = w == 0 ? inlet_update_function : outlet_update_function;
+            for (const auto &p : nodes)
+                for (int k = 0; k < 2; ++k)
+                    if (fns[k]) velocity_tensor(p.coords[0], p.coords[1], (util::Index)k) = fns[k](t, p);
+        }
     }
     static void attach_density_functions(const Fn &inlet_rho, const Fn &outlet_rho)
     {
@@ -226,6 +255,19 @@ class HIPPolicy;
 /// policy describes the whole configuration to the C ABI once
 /// (make_desc) and evaluates the inlet/outlet functions per step
 /// (open_values); Lattice<HIPPolicy<2, C>> then runs the fused step.
+///
+/// It also has the reference policy's statics (OMPPolicy.hpp:50-163), so
+/// a loop written like the reference's perform_lbm (Lattice.hpp:338-410)
+/// runs on the GPU once a context is bound (bind).  The populations stay
+/// on the device, so the tensors the statics receive are not read:
+///  * update_nodes evaluates the update functions into the host velocity
+///    tensor (as the reference does) and into the context;
+///  * the first calc_equilibrium after bind is the run's prologue
+///    (Lattice.hpp:322-328) and initialises the device state;
+///  * collide launches the whole fused time step (lbm_step);
+///  * update_macro, the later calc_equilibrium calls, stream, the open
+///    collide/stream calls and the four boundary updates are work the
+///    fused step has done, and do nothing.
 template <typename Configuration>
     requires IsGloballyHIP<Configuration>
 class HIPPolicy<2, Configuration> : public HIPTag
@@ -236,6 +278,70 @@ public:
     using InletPolicy = typename Configuration::inlet_policy_t;
     using OutletPolicy = typename Configuration::outlet_policy_t;
     using InitializationPolicy = typename Configuration::initialization_policy_t;
+    using Tensor3 = util::HostTensor<double, 3>;
+    using Tensor2 = util::HostTensor<double, 2>;
+
+    /// The context the statics act on, and whether its run has started.
+    static inline lbm_ctx *context = nullptr;
+    static inline bool started = false;
+    /// lbm_step flags of the launches collide makes.
+    static inline uint32_t step_flags = 0;
+
+    static void bind(lbm_ctx *ctx)
+    {
+        context = ctx;
+        started = false;
+    }
+
+    // ----- update of the boundaries (OMPPolicy.hpp:50-72): in the fused step
+    static void update_inlet_boundaries(Tensor3 &, std::vector<BoundaryPoint<2>> &, const Tensor2 &, const Tensor3 &) {}
+    static void update_outlet_boundaries(Tensor3 &, std::vector<BoundaryPoint<2>> &, const Tensor2 &, const Tensor3 &) {}
+    static void update_domain_boundaries(Tensor3 &, std::vector<BoundaryPoint<2>> &, const Tensor2 &, const Tensor3 &) {}
+    static void update_obstacle_boundaries(Tensor3 &, std::vector<ObstaclePoint<2>> &, const Tensor2 &, const Tensor3 &) {}
+
+    // ----- collision and streaming (OMPPolicy.hpp:82-115)
+    /// One whole time step on the bound context.
+    static void collide(const Tensor3 &, Tensor3 &, Tensor3 &, const std::vector<Point<2>> &, Tensor2 &, Tensor3 &,
+                        const double)
+    {
+        require_context();
+        check(lbm_step(context, 1, step_flags), "lbm_step");
+    }
+    static void stream(Tensor3 &, Tensor3 &, const std::vector<Point<2>> &) {}
+    static void collide_open_inlet_boundary(const Tensor3 &, Tensor3 &, Tensor3 &, const std::vector<BoundaryPoint<2>> &,
+                                            Tensor2 &, Tensor3 &, const double)
+    {
+    }
+    static void collide_open_outlet_boundary(const Tensor3 &, Tensor3 &, Tensor3 &, const std::vector<BoundaryPoint<2>> &,
+                                             Tensor2 &, Tensor3 &, const double)
+    {
+    }
+    static void stream_open_inlet_boundary(Tensor3 &, Tensor3 &, const std::vector<BoundaryPoint<2>> &) {}
+    static void stream_open_outlet_boundary(Tensor3 &, Tensor3 &, const std::vector<BoundaryPoint<2>> &) {}
+
+    // ----- initialisation and external updates (OMPPolicy.hpp:127-143)
+    static void attach_nodes(std::vector<BoundaryPoint<2>> &inlet_nodes_, std::vector<BoundaryPoint<2>> &outlet_nodes_)
+    {
+        InitializationPolicy::attach_nodes(inlet_nodes_, outlet_nodes_);
+    }
+    static void update_macro(const Tensor3 &, const std::vector<Point<2>> &, Tensor2 &, Tensor3 &) {}
+    /// Values of step t into the velocity tensor and the bound context.
+    static void update_nodes(const double &t, Tensor3 &velocity_tensor, const Tensor2 &density_tensor)
+    {
+        InitializationPolicy::update_nodes(t, velocity_tensor, density_tensor);
+        if (!context) return;
+        std::vector<double> ux, uy, rho;
+        for (int w = 0; w < 2; ++w) {
+            const auto &nodes = w == 0 ? InitializationPolicy::inlet_nodes : InitializationPolicy::outlet_nodes;
+            if (nodes.empty()) continue;
+            open_values(t, w, nodes, ux, uy, rho);
+            check(lbm_set_open_values(context, w, ux.data(), uy.data(), rho.data()), "lbm_set_open_values");
+        }
+    }
+
+    // ----- equilibrium (OMPPolicy.hpp:153-163)
+    static void calc_equilibrium(std::vector<Point<2>> &, Tensor3 &, Tensor3 &, Tensor2 &) { prologue(); }
+    static void calc_equilibrium(std::vector<BoundaryPoint<2>> &, Tensor3 &, Tensor3 &, Tensor2 &) { prologue(); }
 
     static lbm_desc make_desc(util::Index nr, util::Index nc, int32_t storage = LBM_STORE_F64, int32_t device = 0)
     {
@@ -268,6 +374,23 @@ public:
             uy[q] = fns[1] ? fns[1](t, nodes[q]) : 0.0;
             rho[q] = rfn ? rfn(t, nodes[q]) : 1.0;
         }
+    }
+
+private:
+    static void require_context()
+    {
+        if (!context) throw std::logic_error("HIPPolicy: no context bound (HIPPolicy::bind)");
+    }
+    static void check(int code, const char *call)
+    {
+        if (code != LBM_OK) throw std::runtime_error(std::string(call) + ": " + lbm_last_error());
+    }
+    static void prologue()
+    {
+        require_context();
+        if (started) return;
+        check(lbm_init_equilibrium(context), "lbm_init_equilibrium");
+        started = true;
     }
 };
 

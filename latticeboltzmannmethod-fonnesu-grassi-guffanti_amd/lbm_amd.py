@@ -31,13 +31,15 @@ STEP_MACRO, STEP_CHECK, STEP_NO_GRAPH = 1, 2, 4
 OPT_F16_ROUND_TO_NEAREST = 1
 # lbm_set_launch shape: one wave per run of up to 64 consecutive wet rows
 LAUNCH_WET_RUNS = 64
-ERR_NAMES = {0: "OK", 1: "INVALID", 2: "HIP", 3: "NAN", 4: "STATE", 5: "UNSUPPORTED", 6: "NOMEM"}
+ERR_NAMES = {0: "OK", 1: "INVALID", 2: "HIP", 3: "NAN", 4: "STATE", 5: "UNSUPPORTED", 6: "NOMEM", 7: "RANGE"}
+ERR_INVALID, ERR_STATE, ERR_NAN, ERR_UNSUPPORTED, ERR_RANGE = 1, 4, 3, 5, 7
 
 EXPORTED = [
     "lbm_version", "lbm_last_error", "lbm_create", "lbm_destroy", "lbm_set_relaxation", "lbm_set_geometry",
     "lbm_node_counts", "lbm_open_nodes", "lbm_set_open_values", "lbm_init_equilibrium",
     "lbm_upload_populations", "lbm_download_populations", "lbm_download_after_collision", "lbm_step",
-    "lbm_get_time", "lbm_download_macro", "lbm_lift_drag", "lbm_reduce", "lbm_synchronize", "lbm_get_stream",
+    "lbm_get_time", "lbm_download_macro", "lbm_download_macro_async", "lbm_wait", "lbm_host_alloc",
+    "lbm_host_free", "lbm_lift_drag", "lbm_lift_drag_async", "lbm_reduce", "lbm_synchronize", "lbm_get_stream",
     "lbm_get_scale", "lbm_geometry_cavity", "lbm_geometry_channel", "lbm_geometry_add_disc",
     "lbm_geometry_porous", "lbm_set_launch", "lbm_get_launch", "lbm_autotune", "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_set_launch", "lbm_batch_get_launch",
     "lbm_batch_autotune", "lbm_batch_step", "lbm_batch_reduce", "lbm_batch_info", "lbm_batch_get_stream",
@@ -95,7 +97,12 @@ def lib():
             "lbm_step": [vp, C.c_int64, C.c_uint32],
             "lbm_get_time": [vp, i64p],
             "lbm_download_macro": [vp, dp, dp],
+            "lbm_download_macro_async": [vp, dp, dp, i64p],
+            "lbm_wait": [vp, C.c_int64],
+            "lbm_host_alloc": [C.c_int64, C.POINTER(vp)],
+            "lbm_host_free": [vp],
             "lbm_lift_drag": [vp, i64p, i32p, C.c_int64, dp, dp],
+            "lbm_lift_drag_async": [vp, i64p, i32p, C.c_int64, dp, dp, i64p],
             "lbm_reduce": [vp, dp],
             "lbm_synchronize": [vp],
             "lbm_get_stream": [vp, C.POINTER(vp)],
@@ -275,6 +282,19 @@ class Lattice:
         u = np.empty((2, self.nc, self.nr))
         _check(lib().lbm_download_macro(self._h, _dp(rho), _dp(u)))
         return rho, u
+
+    def macro_async(self, rho, u):
+        """Queues the download of the last LBM_STEP_MACRO step's fields into
+        rho (NC, NR) and u (2, NC, NR) (float64, C-contiguous, kept alive by
+        the caller until wait(ticket)); returns the ticket."""
+        assert rho.shape == (self.nc, self.nr) and u.shape == (2, self.nc, self.nr)
+        assert rho.flags.c_contiguous and u.flags.c_contiguous and rho.dtype == u.dtype == np.float64
+        t = C.c_int64(0)
+        _check(lib().lbm_download_macro_async(self._h, _dp(rho), _dp(u), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket):
+        _check(lib().lbm_wait(self._h, int(ticket)))
 
     def lift_drag(self, points=None, masks=None, per_point=False):
         ld = np.zeros(2)
