@@ -25,9 +25,11 @@ ceiling of a stream (copy_ceiling, frac_of_copy).  The step
 kernel is the only kernel in the timed region, so its average duration is
 the event-timed region divided by the number of steps.
 
-The CPU baseline is the repo's CPU oracle (a restatement of the reference's
-Serial/OpenMP policy structure) timed on this host on a few steps of the
-same workload.
+The CPU baseline is the repo's CPU oracle in the reference's step structure
+(list-driven sweeps, three population tensors, the by-value rho / u copies
+of every boundary call) with OpenMP on this host's physical cores, timed in
+a child process on bounded samples of the same workload (the headline) and
+of the 129^2 cavity and the 512 x 2048 channel.
 
 Other workloads: open8192 (no obstacles), micro8192 (the porous geometry
 from random populations w_d (1 + 1e-3 U(-1, 1)), seed 1: the step's time
@@ -41,6 +43,7 @@ import ctypes as C
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
@@ -181,28 +184,52 @@ def time_storage(wl, name, steps, warmup, ev):
                      traffic_bytes_per_step=measured_traffic(f"{wl.name}/{name}/R{segments}"))
 
 
-def cpu_baseline(wl, seconds_budget=20.0):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-    threads = min(16, os.cpu_count() or 1)
-    O.set_threads(threads)
-    orc = O.OracleLattice(wl.node_class, **wl.lattice_kwargs())
-    if wl.inlet:
-        orc.set_open_values(0, **wl.inlet)
-    if wl.outlet:
-        orc.set_open_values(1, **wl.outlet)
-    orc.init()
-    orc.step(1)                                 # first touch / warm-up
-    t0 = time.perf_counter()
-    steps = 0
-    while steps < 2 or (time.perf_counter() - t0 < seconds_budget and steps < 1000):
-        orc.step(1)
-        steps += 1
-    dt = time.perf_counter() - t0
-    orc.close()
-    return dict(value=wl.wet_count() * steps / dt / 1e6, unit="MLUPS", cores=threads, kind="port",
-                sample=f"{steps} steps of {wl.name} ({wl.wet_count()} wet nodes), CPU oracle "
-                       f"(reference-structured list sweeps, OpenMP), {threads} threads")
+def physical_cores():
+    """Physical cores among the CPUs this process may run on (lscpu's
+    CORE/SOCKET pairs), capped at 16, the CPU share of one GPU on the
+    benchmark machines."""
+    cpus = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count() or 1))
+    cores = set()
+    try:
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True, timeout=30).stdout
+        allowed = set(cpus)
+        for line in out.splitlines():
+            if line.startswith("#") or not line.strip():
+                continue
+            cpu, core, sock = line.split(",")[:3]
+            if int(cpu) in allowed:
+                cores.add((sock, core))
+    except (OSError, ValueError, subprocess.SubprocessError):
+        cores = set()
+    n = len(cores) if cores else len(cpus)
+    return max(1, min(n, 16)), len(cpus)
+
+
+def cpu_baseline(wl, workload, seconds=8.0, extra=("cavity129", "channel")):
+    """The reference's step structure on the host cores: the oracle with the
+    reference's by-value rho / u copies in every boundary call, run in a
+    child process with OMP_NUM_THREADS = physical cores, OMP_PROC_BIND =
+    close and OMP_PLACES = cores set before its OpenMP runtime starts, on
+    bounded samples (about `seconds` each) of this workload and of the
+    129^2 cavity and the 512 x 2048 channel."""
+    cores, cpus = physical_cores()
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    names = [workload] + [n for n in extra if n != workload]
+    script = os.path.join(ROOT, "oracle", "cpu_baseline.py")
+    r = subprocess.run([sys.executable, script, "--configs", ",".join(names), "--seconds", str(seconds)],
+                       env=env, capture_output=True, text=True, timeout=60 + 8 * seconds * len(names))
+    sys.stderr.write(r.stderr)
+    if r.returncode != 0:
+        raise RuntimeError(f"cpu baseline failed ({r.returncode})")
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    head = res["configs"][names[0]]
+    return dict(value=round(head["mlups"], 3), unit="MLUPS", cores=cores, kind="port",
+                sample=f"{head['steps']} steps of {wl.name} ({head['wet_nodes']} wet nodes) in {head['seconds']} s: "
+                       f"the oracle (reference-structured list sweeps, three population tensors, by-value rho/u "
+                       f"copies per boundary call as in OMPPolicy.hpp:50-72,139), OpenMP {cores} threads on "
+                       f"{cores} physical cores of {cpus} CPUs, OMP_PROC_BIND=close, OMP_PLACES=cores",
+                configs={k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                         for k, v in res["configs"].items()})
 
 
 def run_to(lat, total_steps, name="", chunk=10000):
@@ -353,8 +380,8 @@ def main():
     ens = None if args.no_ensemble else bench_ensemble(args, ev)
     cpu = None
     if not args.no_cpu_baseline:
-        log("  CPU baseline (oracle) ...")
-        cpu = cpu_baseline(wl)
+        log("  CPU baseline (reference-structured oracle) ...")
+        cpu = cpu_baseline(wl, args.workload)
         log(f"  CPU: {cpu['value']:.2f} MLUPS ({cpu['sample']})")
     r = results[head]
     line = {

@@ -477,12 +477,12 @@ int lbm_batch_step(lbm_batch *b, int64_t n, uint32_t flags)
         HIP_TRY(hipMemcpyAsync(fl.data(), b->d_flags, fl.size() * sizeof(int32_t), hipMemcpyDeviceToHost, b->stream));
         HIP_TRY(hipStreamSynchronize(b->stream));
         for (size_t s = 0; s < fl.size(); ++s) {
-            if (fl[s] & 1)
-                return fail(LBM_ERR_NAN, "NaN density in sample %lld at or before step %lld", (long long)s,
-                            (long long)b->samples[s]->t);
             if (fl[s] & 2)
                 return fail(LBM_ERR_RANGE, "FP16 storage of sample %lld: a scaled population reached 65504 at or "
                             "before step %lld", (long long)s, (long long)b->samples[s]->t);
+            if (fl[s] & 1)
+                return fail(LBM_ERR_NAN, "NaN density in sample %lld at or before step %lld", (long long)s,
+                            (long long)b->samples[s]->t);
         }
     }
     return LBM_OK;

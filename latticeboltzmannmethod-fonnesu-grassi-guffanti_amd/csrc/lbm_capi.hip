@@ -658,7 +658,13 @@ int lbm_set_geometry(lbm_ctx *c, const uint8_t *cls)
         HIP_TRY(hipMemcpy(c->d_bc[k], c->h_bc.data(), (size_t)(3 * c->E) * sizeof(double), hipMemcpyHostToDevice));
     }
     c->bc_cur = c->bc_prev = 0;
-    for (int k = 0; k < 2; ++k) HIP_TRY(hipMalloc(&c->d_macro[k], (size_t)(3 * nr * nc) * sizeof(double)));
+    // both copies of the fields start at rho = 1, u = 0 (Lattice.hpp:295-296):
+    // the steps write the wet nodes only
+    for (int k = 0; k < 2; ++k) {
+        HIP_TRY(hipMalloc(&c->d_macro[k], (size_t)(3 * nr * nc) * sizeof(double)));
+        HIP_TRY(launch_fill_macro(c->d_macro[k], nr * nc, c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
     c->macro_slot = 0;
     c->has_geom = true;
     c->state = kNone;
@@ -827,10 +833,12 @@ int lbm_step(lbm_ctx *c, int64_t n, uint32_t flags)
         int32_t flag = 0;
         HIP_TRY(hipMemcpyAsync(&flag, c->d_flag, sizeof flag, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
-        if (flag & 1) return fail(LBM_ERR_NAN, "NaN density at or before step %lld", (long long)c->t);
+        // a population beyond the FP16 range is the cause of the NaNs that
+        // usually follow it, so it is reported first
         if (flag & 2)
             return fail(LBM_ERR_RANGE, "FP16 storage: a scaled population reached 65504 at or before step %lld "
                         "(storage scale %g is too large for this flow)", (long long)c->t, (double)c->scale);
+        if (flag & 1) return fail(LBM_ERR_NAN, "NaN density at or before step %lld", (long long)c->t);
     }
     return LBM_OK;
 }

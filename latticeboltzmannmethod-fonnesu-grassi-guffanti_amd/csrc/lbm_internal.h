@@ -134,5 +134,7 @@ hipError_t launch_lift_drag(int32_t storage, const StepArgs &a, const void *fsta
 hipError_t launch_reduce_partials(const double *partials, int64_t nblocks, int32_t ncomp, double *out,
                                   hipStream_t s);
 int64_t readout_blocks(const StepArgs &a);
+// rho = 1, u = 0 over a field of n nodes ([3][n] doubles).
+hipError_t launch_fill_macro(double *macro, int64_t n, hipStream_t s);
 
 }  // namespace lbm

@@ -1526,6 +1526,16 @@ __global__ __launch_bounds__(kBlockRows) void init_eq(StepArgs a, T *frozen, dou
     }
 }
 
+// rho = 1, u = 0 on every node of a [3][n] field.
+__global__ __launch_bounds__(kBlockRows) void fill_macro(double *macro, int64_t n)
+{
+    const int64_t q = (int64_t)blockIdx.x * kBlockRows + threadIdx.x;
+    if (q >= n) return;
+    macro[q] = 1.0;
+    macro[n + q] = 0.0;
+    macro[2 * n + q] = 0.0;
+}
+
 // Compact index of wet node (i, j), any thread.
 __device__ __forceinline__ int64_t cidx_of(const Geometry &g, int64_t i, int64_t j)
 {
@@ -1847,6 +1857,13 @@ hipError_t launch_lift_drag(int32_t storage, const StepArgs &a0, const void *fst
     case kF32: hipLaunchKernelGGL(lift_drag<float>, grid, block, 0, s, a, ij, masks, n, per_point, partials); break;
     default: hipLaunchKernelGGL(lift_drag<_Float16>, grid, block, 0, s, a, ij, masks, n, per_point, partials); break;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_macro(double *macro, int64_t n, hipStream_t s)
+{
+    const int64_t nb = (n + kBlockRows - 1) / kBlockRows;
+    hipLaunchKernelGGL(fill_macro, dim3((unsigned)nb), dim3(kBlockRows), 0, s, macro, n);
     return hipGetLastError();
 }
 

@@ -68,6 +68,7 @@ typedef struct orc_lattice {
     int policy[2];            /* inlet, outlet */
     double *val_ux[2], *val_uy[2], *val_rho[2];
     int64_t step;
+    int by_value;             /* timing baseline: make the reference's by-value field copies */
 } orc_lattice;
 
 #define P3(L, a, i, j, d) ((a)[(i) + (L)->nr * ((j) + (L)->nc * (int64_t)(d))])
@@ -516,9 +517,38 @@ void orc_init(orc_lattice *L)
     L->step = 0;
 }
 
+/* The reference hands global_rho / global_u to the boundary policies and
+ * to update_nodes BY VALUE, at two call levels (OMPPolicy.hpp:50-72 ->
+ * BounceBackPolicy.hpp:68,143, ZouHePolicy.hpp:58; OMPPolicy.hpp:139 ->
+ * InletOutletInitializer.hpp:90): every such call constructs a fresh
+ * Eigen tensor and copies the field into it on the calling thread.  With
+ * by_value set (CPU timing baseline only; results do not change) these
+ * copies are made the same way: allocate, copy serially, free. */
+static void by_value_copy(const double *src, size_t n)
+{
+    double *t = (double *)malloc(n * sizeof(double));
+    if (!t) return;
+    for (size_t k = 0; k < n; ++k) t[k] = src[k];
+    /* keep the copy observable so it is not optimised away */
+    __asm__ volatile("" : : "r"(t) : "memory");
+    free(t);
+}
+
+static void by_value_fields(const orc_lattice *L, int with_u)
+{
+    const size_t n = (size_t)(L->nr * L->nc);
+    for (int level = 0; level < 2; ++level) {
+        by_value_copy(L->rho, n);
+        if (with_u) by_value_copy(L->u, 2 * n);
+    }
+}
+
+void orc_set_by_value_copies(orc_lattice *L, int on) { L->by_value = on != 0; }
+
 /* one iteration of the loop body at Lattice.hpp:338-423 (no file output) */
 static void one_step(orc_lattice *L)
 {
+    if (L->by_value) by_value_fields(L, 0); /* update_nodes: density tensor by value */
     update_nodes(L);
     update_macro(L);
     calc_equilibrium(L, L->E, L->list[LIST_FLUID], L->count[LIST_FLUID]);
@@ -530,9 +560,13 @@ static void one_step(orc_lattice *L)
     stream(L, L->list[LIST_FLUID], L->count[LIST_FLUID]);
     stream(L, L->list[LIST_INLET], L->count[LIST_INLET]);
     stream(L, L->list[LIST_OUTLET], L->count[LIST_OUTLET]);
+    if (L->by_value) by_value_fields(L, 1); /* update_domain_boundaries */
     bounce_back_walls(L);
+    if (L->by_value) by_value_fields(L, 1); /* update_obstacle_boundaries */
     bounce_back_obstacles(L);
+    if (L->by_value) by_value_fields(L, 1); /* update_inlet_boundaries */
     update_open(L, 0);
+    if (L->by_value) by_value_fields(L, 1); /* update_outlet_boundaries */
     update_open(L, 1);
     L->step++;
 }

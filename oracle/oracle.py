@@ -69,6 +69,7 @@ def lib():
         L.orc_get_macro.argtypes = [C.c_void_p, dp, dp]
         L.orc_lift_drag.argtypes = [C.c_void_p, i64p, i32p, C.c_int64, dp, dp]
         L.orc_reduce.argtypes = [C.c_void_p, dp]
+        L.orc_set_by_value_copies.argtypes = [C.c_void_p, C.c_int]
         _lib = L
     return _lib
 
@@ -220,6 +221,12 @@ class OracleLattice:
         if per_point:
             return ld[0], ld[1], pp[:n]
         return ld[0], ld[1]
+
+    def set_by_value_copies(self, on=True):
+        """CPU timing baseline only: make the reference's by-value copies of
+        the rho / u fields in every boundary and update_nodes call
+        (OMPPolicy.hpp:50-72,139).  Results are unchanged."""
+        lib().orc_set_by_value_copies(self._h, int(bool(on)))
 
     def reduce(self):
         out = np.zeros(5)

@@ -86,10 +86,11 @@ def test_cavity_trt_shifted_storage_errors(name, storage, opts, bu, br):
 @pytest.mark.slow
 def test_cavity_trt_4096_full_size():
     """The 4096 x 4096 configuration itself in all three storage formats.
-    (a) Lid at rest: after 100 steps the momentum is exactly zero in every
-    format; shifted storage holds exact zeros, so its mass is bit for bit
-    the mass after the initialisation, and FP64 (whose nine weights sum to
-    1 + 2^-52, not 1) keeps it to 1e-12.  (b) Lid at 0.1: after 300
+    (a) Lid at rest: shifted storage holds exact zeros, so after 100 steps
+    its momentum is exactly zero and its mass bit for bit the mass after
+    the initialisation; FP64 (whose nine weights sum to 1 + 2^-52, not 1,
+    so the rest state moves by roundings) keeps the mass to 1e-12 and the
+    momentum below 1e-9.  (b) Lid at 0.1: after 300
     steps the FP32 / FP16 fields are within the stated L2 bounds of the
     FP64 run (velocity: FP32 1e-5, FP16 stochastic 5e-3) and the FP64
     reduced mass agrees to 1e-6 relative."""
@@ -107,12 +108,13 @@ def test_cavity_trt_4096_full_size():
         red = lat.reduce()
         say(f"  storage {storage} at rest: mass {red[0]!r} (after init {m0!r}, wet {nw}), "
             f"momentum {red[1]!r}, {red[2]!r}")
-        assert red[1] == 0.0 and red[2] == 0.0
         assert abs(m0 - nw) <= 1e-12 * nw
         if storage == L.F64:
             assert abs(red[0] - m0) <= 1e-12 * nw
+            assert abs(red[1]) <= 1e-9 and abs(red[2]) <= 1e-9
         else:
             assert red[0] == m0
+            assert red[1] == 0.0 and red[2] == 0.0
         lat.close()
     runs = {}
     for storage in (L.F64, L.F32_SHIFTED, L.F16_SHIFTED):
@@ -341,8 +343,11 @@ def test_async_macro_downloads_match_sync():
         tickets.append(a.macro_async(rho, u))
         bufs.append((rho, u))
         want.append(b.macro())
+    wall = wl.node_class == L.BOUNDARY
     for t, (rho, u), (rw, uw) in zip(tickets, bufs, want):
         a.wait(t)
         assert np.array_equal(rho, rw) and np.array_equal(u, uw)
+        # nodes the steps never write hold rho = 1, u = 0 (Lattice.hpp:295-296)
+        assert np.all(rho[wall] == 1.0) and np.all(u[:, wall] == 0.0)
     with pytest.raises(L.LbmError):
         a.wait(tickets[-1] + 5)
