@@ -239,6 +239,20 @@ int lbm_lift_drag_async(lbm_ctx *ctx, const int64_t *ij, const int32_t *masks, i
  * FlowAnalyzer); used for mass drift and Darcy permeability. */
 int lbm_reduce(lbm_ctx *ctx, double out[5]);
 
+/* lbm_reduce without allocating and without waiting: the five sums go to
+ * `out`, 5 doubles in DEVICE memory, in stream order on the context's
+ * stream (lbm_get_stream), with the same kernels and the same bits as
+ * lbm_reduce.  No counterpart in the reference: for callers that cannot
+ * allocate -- code under HIP graph capture, tight serving loops, hosts
+ * that forbid hipMalloc.  The block partials live in one fixed scratch
+ * array per device (5 MiB, held until the process ends), so the lattice
+ * may have at most 131072 reduction blocks, ceil(NR / 256) * NC (e.g.
+ * 4096 x 8192); larger ones are LBM_ERR_UNSUPPORTED.  All calls on one
+ * device share that array: calls whose kernels can overlap (contexts on
+ * different streams not ordered by the caller, or threads) race, and the
+ * caller must order them. */
+int lbm_reduce_device(lbm_ctx *ctx, double *out);
+
 /* Waits for all work queued on the context. */
 int lbm_synchronize(lbm_ctx *ctx);
 

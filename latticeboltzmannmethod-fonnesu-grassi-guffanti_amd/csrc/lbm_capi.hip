@@ -16,6 +16,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -232,6 +233,40 @@ int64_t edge_slot_host(int64_t i, int64_t j, int64_t nr, int64_t nc)
 const int kOffI[9] = {0, 0, -1, 0, 1, -1, -1, 1, 1};
 const int kOffJ[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
 
+// Scratch of the allocation-free reduction (lbm_reduce_device): the
+// per-block partial sums of the readout kernel, 5 doubles per block of
+// 256 rows x 1 column, for lattices of at most kScratchBlocks such blocks
+// (ceil(NR / 256) * NC; e.g. 4096 x 8192).  Every device the library runs
+// on holds its own copy (5 MiB) until the process ends.  Host code never
+// passes the array's name to a kernel: it uses the device address that
+// hipGetSymbolAddress returned for the current device (scratch_ptr).
+constexpr int64_t kScratchBlocks = int64_t(1) << 17;
+static __device__ __align__(256) double scratch[5 * kScratchBlocks];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+
+// Looks the scratch array up on `device` (which must be current) once per
+// device; called by lbm_create.
+int init_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(LBM_ERR_UNSUPPORTED, "device ordinal %d >= %d", device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return LBM_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return fail(LBM_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch[device] = (double *)ptr;
+    return LBM_OK;
+}
+
+double *scratch_ptr(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return (device >= 0 && device < kMaxDevices) ? g_scratch[device] : nullptr;
+}
+
 bool wet_at(const lbm_ctx *c, int64_t i, int64_t j)
 {
     return (c->h_wet[(size_t)((j + 1) * c->wl + 1 + i / 64)] >> (i % 64)) & 1;
@@ -373,6 +408,10 @@ int lbm_create(const lbm_desc *desc, lbm_ctx **out)
     HIP_TRY(hipGetDeviceCount(&ndev));
     if (desc->device < 0 || desc->device >= ndev) return fail(LBM_ERR_INVALID, "no HIP device %d", desc->device);
     HIP_TRY(hipSetDevice(desc->device));
+    {
+        const int r = init_scratch(desc->device);
+        if (r) return r;
+    }
     lbm_ctx *c = new lbm_ctx();
     c->desc = *desc;
     c->storage = desc->storage;
@@ -1017,6 +1056,25 @@ int lbm_reduce(lbm_ctx *c, double out[5])
     HIP_TRY(launch_reduce_partials(c->d_partials, nb, 5, c->d_out, c->stream));
     HIP_TRY(hipMemcpyAsync(out, c->d_out, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_reduce_device(lbm_ctx *c, double *out)
+{
+    if (!c || !out) return fail(LBM_ERR_INVALID, "null argument");
+    if (c->state == kNone) return fail(LBM_ERR_STATE, "no populations");
+    HIP_TRY(hipSetDevice(c->desc.device));
+    StepArgs a = make_args(c);
+    a.fin = c->d_F[c->cur];
+    const int64_t nb = readout_blocks(a);
+    if (nb > kScratchBlocks)
+        return fail(LBM_ERR_UNSUPPORTED, "lattice needs %lld reduction blocks (ceil(NR / 256) * NC); the "
+                    "allocation-free reduction holds %lld", (long long)nb, (long long)kScratchBlocks);
+    double *partials = scratch_ptr(c->desc.device);
+    if (!partials) return fail(LBM_ERR_STATE, "scratch of device %d not set up", c->desc.device);
+    // the same kernels and partial layout as lbm_reduce: the same bits
+    HIP_TRY(launch_readout(c->storage, c->state == kFstar, a, nullptr, partials, c->stream));
+    HIP_TRY(launch_reduce_partials(partials, nb, 5, out, c->stream));
     return LBM_OK;
 }
 

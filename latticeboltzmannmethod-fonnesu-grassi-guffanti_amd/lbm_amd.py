@@ -39,7 +39,8 @@ EXPORTED = [
     "lbm_node_counts", "lbm_open_nodes", "lbm_set_open_values", "lbm_init_equilibrium",
     "lbm_upload_populations", "lbm_download_populations", "lbm_download_after_collision", "lbm_step",
     "lbm_get_time", "lbm_download_macro", "lbm_download_macro_async", "lbm_wait", "lbm_host_alloc",
-    "lbm_host_free", "lbm_lift_drag", "lbm_lift_drag_async", "lbm_reduce", "lbm_synchronize", "lbm_get_stream",
+    "lbm_host_free", "lbm_lift_drag", "lbm_lift_drag_async", "lbm_reduce", "lbm_reduce_device", "lbm_synchronize",
+    "lbm_get_stream",
     "lbm_get_scale", "lbm_geometry_cavity", "lbm_geometry_channel", "lbm_geometry_add_disc",
     "lbm_geometry_porous", "lbm_set_launch", "lbm_get_launch", "lbm_autotune", "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_set_launch", "lbm_batch_get_launch",
     "lbm_batch_autotune", "lbm_batch_step", "lbm_batch_reduce", "lbm_batch_info", "lbm_batch_get_stream",
@@ -104,6 +105,7 @@ def lib():
             "lbm_lift_drag": [vp, i64p, i32p, C.c_int64, dp, dp],
             "lbm_lift_drag_async": [vp, i64p, i32p, C.c_int64, dp, dp, i64p],
             "lbm_reduce": [vp, dp],
+            "lbm_reduce_device": [vp, vp],
             "lbm_synchronize": [vp],
             "lbm_get_stream": [vp, C.POINTER(vp)],
             "lbm_get_scale": [vp, dp],
@@ -317,6 +319,11 @@ class Lattice:
         out = np.zeros(5)
         _check(lib().lbm_reduce(self._h, _dp(out)))
         return out
+
+    def reduce_device(self, out_ptr):
+        """lbm_reduce_device: queues the five sums into 5 doubles of device
+        memory at out_ptr (no allocation, no wait)."""
+        _check(lib().lbm_reduce_device(self._h, C.c_void_p(int(out_ptr))))
 
     def synchronize(self):
         _check(lib().lbm_synchronize(self._h))
