@@ -267,6 +267,36 @@ def stated_errors(wl, lat_ref, lats, total_steps):
     return {"steps": total_steps, "vs": "f64 GPU path (oracle-pinned)", "by_storage": out}
 
 
+def bench_cavity_trt(args, ev):
+    """The lid-driven TRT cavity at scale (4096 x 4096, Re 1000, magic 3/16):
+    MLUPS and roofline fraction of the FP64, FP32 and FP16 storage formats
+    over the same step count as the headline, then the FP32/FP16 errors
+    against the FP64 run after `args.c4_error_steps` steps (the
+    configuration's 200,000 by default)."""
+    wl = W.cavity_trt(4096)
+    log(f"cavity_trt4096: {wl.wet_count()} wet nodes, TRT tau+ {wl.tau_even:.4f}, tau- {wl.tau_odd:.4f}")
+    results, lats = {}, {}
+    for s in ("f64", "f32", "f16"):
+        lat, r = time_storage(wl, s, args.steps, args.warmup, ev)
+        results[s] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+        lats[s] = lat
+        log(f"  cavity_trt4096 {s}: {r['mlups']:.1f} MLUPS, {r['ms_per_step']:.4f} ms/step, "
+            f"{r['achieved_gbs']:.0f} GB/s = {100 * r['frac']:.1f}% of 8 TB/s (launch {r['segments']})")
+    errors = None
+    if args.c4_error_steps > 0:
+        t1 = time.perf_counter()
+        errors = stated_errors(wl, lats["f64"], {s: lat for s, lat in lats.items() if s != "f64"},
+                               args.c4_error_steps)
+        for name, e in errors["by_storage"].items():
+            log(f"  cavity_trt4096 {name} vs f64 after {args.c4_error_steps} steps: velocity L2 rel "
+                f"{e['velocity_l2_rel']:.3e}, density L2 rel {e['density_l2_rel']:.3e}, mass rel diff "
+                f"{e['mass_rel_diff']:.3e} ({time.perf_counter() - t1:.1f}s)")
+    for lat in lats.values():
+        lat.close()
+    return dict(workload=wl.name, wet_nodes=wl.wet_count(), collision="TRT", re=wl.info["re"],
+                u_lid=wl.info["u_lid"], storages=results, error_vs_f64=errors)
+
+
 def bench_ensemble(args, ev):
     """Many independent porous lattices of uneven sizes advanced by one
     persistent launch per step (lbm_batch), HIP-graph step loop, tile shape
@@ -339,6 +369,9 @@ def main():
     ap.add_argument("--ensemble-check-steps", type=int, default=10000,
                     help="steps after which the ensemble's FP32 permeabilities are checked against FP64 (0 = skip)")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-c4", action="store_true", help="skip the 4096^2 TRT cavity line")
+    ap.add_argument("--c4-error-steps", type=int, default=200000,
+                    help="steps after which the 4096^2 TRT cavity's FP32/FP16 errors are stated (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -378,6 +411,7 @@ def main():
     for lat in lats.values():
         lat.close()
     ens = None if args.no_ensemble else bench_ensemble(args, ev)
+    c4 = None if args.no_c4 else bench_cavity_trt(args, ev)
     cpu = None
     if not args.no_cpu_baseline:
         log("  CPU baseline (reference-structured oracle) ...")
@@ -408,6 +442,7 @@ def main():
                      for s, x in results.items()},
         "error_vs_f64": errors,
         "ensemble": ens,
+        "cavity_trt4096": c4,
     }
     if "f64" in results and head != "f64":
         line["speedup_vs_f64"] = round(r["mlups"] / results["f64"]["mlups"], 3)
