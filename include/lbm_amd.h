@@ -119,6 +119,23 @@ int lbm_destroy(lbm_ctx *ctx);
  * TRTCollision.hpp:128-134). */
 int lbm_set_relaxation(lbm_ctx *ctx, double tau, double tau_even, double tau_odd);
 
+/* Makes the wet obstacles partially saturated bounce-back nodes
+ * (PSBounceBackPolicy<2> as the lattice's obstacle policy,
+ * PSBounceBackPolicy.hpp:177-297): each obstacle o with weight b sends
+ * into its wet neighbour n = o + c_d
+ *   p_q - (1 - b) (p_q - e_q) / tau + b ((P_d(n) - e_d(n)) - (p_q - 2 w_d))
+ * with q = opp d, p_q = F*_q(n), e the equilibrium of n, instead of the
+ * plain bounce-back p_q.  tau is the policy's own relaxation time
+ * (PSBounceBackPolicy<2>::tau, set by its initialize / allowed_tau);
+ * b[k] is the weight of the k-th wet obstacle in list order
+ * (ObstaclePoint<2>::b, Lattice.hpp:245-278), NULL = 1 for every obstacle
+ * (the value the reference's weights keep).  tau <= 0 switches it off.
+ * FP64 storage only (LBM_ERR_UNSUPPORTED otherwise); every node then
+ * takes the general path of the step.  Set after lbm_set_geometry and
+ * before the first step of a run (LBM_ERR_STATE otherwise); not for
+ * ensemble samples. */
+int lbm_set_partial_bounce_back(lbm_ctx *ctx, double tau, const double *b);
+
 /* Installs the geometry from a class plane (ColMajor NR*NC).  Replaces
  * Lattice::reinit (Lattice.hpp:453-481) with the node lists that
  * build_lattice / read_lattice_file produce (ConstructionTools.hpp:54-201,

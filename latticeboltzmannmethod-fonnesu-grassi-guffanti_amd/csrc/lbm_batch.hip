@@ -281,6 +281,9 @@ int lbm_batch_create(lbm_ctx *const *samples, int64_t n, lbm_batch **out)
         if (c->batch) return fail(LBM_ERR_STATE, "sample %lld already belongs to an ensemble", (long long)s);
         if (!c->has_geom) return fail(LBM_ERR_STATE, "sample %lld has no geometry", (long long)s);
         if (c->storage != samples[0]->storage) return fail(LBM_ERR_INVALID, "samples differ in storage format");
+        if (c->psbb)
+            return fail(LBM_ERR_UNSUPPORTED, "sample %lld uses partially saturated bounce-back (single lattices only)",
+                        (long long)s);
         if (c->desc.device != samples[0]->desc.device) return fail(LBM_ERR_INVALID, "samples are on different devices");
         for (int64_t q = 0; q < s; ++q)
             if (samples[q] == c) return fail(LBM_ERR_INVALID, "sample %lld appears twice", (long long)s);

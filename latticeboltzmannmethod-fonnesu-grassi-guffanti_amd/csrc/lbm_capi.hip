@@ -84,6 +84,11 @@ StepArgs make_args(const lbm_ctx *c)
     a.nruns = c->nruns;
     a.perim = c->d_perim;
     a.nperim = c->nperim;
+    a.psbb = c->psbb ? 1 : 0;
+    a.ps_t = c->psbb ? 1.0 / c->ps_tau : 0.0;
+    a.ps_mask = c->d_ps_mask;
+    a.ps_b = c->d_ps_b;
+    a.ps_val = c->d_ps_val;
     return a;
 }
 
@@ -128,6 +133,15 @@ int capture_steps(lbm_ctx *c)
     return LBM_OK;
 }
 
+void free_psbb(lbm_ctx *c)
+{
+    (void)hipFree(c->d_ps_mask); c->d_ps_mask = nullptr;
+    (void)hipFree(c->d_ps_b); c->d_ps_b = nullptr;
+    (void)hipFree(c->d_ps_val); c->d_ps_val = nullptr;
+    c->psbb = false;
+    c->ps_tau = 0.0;
+}
+
 void free_geometry(lbm_ctx *c)
 {
     drop_graphs(c);
@@ -155,6 +169,7 @@ void free_geometry(lbm_ctx *c)
     c->h_pts.clear();
     c->h_masks.clear();
     (void)hipFree(c->d_staging); c->d_staging = nullptr;
+    free_psbb(c);
     c->has_geom = false;
     c->state = kNone;
 }
@@ -481,6 +496,58 @@ int lbm_set_relaxation(lbm_ctx *c, double tau, double tau_even, double tau_odd)
     return LBM_OK;
 }
 
+int lbm_set_partial_bounce_back(lbm_ctx *c, double tau, const double *b)
+{
+    if (!c) return fail(LBM_ERR_INVALID, "null context");
+    if (!c->has_geom) return fail(LBM_ERR_STATE, "no geometry");
+    if (c->batch) return fail(LBM_ERR_STATE, "context belongs to an ensemble");
+    if (c->state == kFstar)
+        return fail(LBM_ERR_STATE, "partially saturated bounce-back is set before the run (after the geometry, "
+                    "before or right after lbm_init_equilibrium / lbm_upload_populations)");
+    if (c->storage != kF64 && tau > 0)
+        return fail(LBM_ERR_UNSUPPORTED, "partially saturated bounce-back needs FP64 storage");
+    const int64_t nob = (int64_t)c->obst_mask.size();
+    if (tau > 0 && b)
+        for (int64_t q = 0; q < nob; ++q)
+            if (!std::isfinite(b[q])) return fail(LBM_ERR_INVALID, "weight of obstacle %lld is not finite", (long long)q);
+    if (tau > 0 && !std::isfinite(tau)) return fail(LBM_ERR_INVALID, "tau must be finite");
+    HIP_TRY(hipSetDevice(c->desc.device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->version++;
+    drop_graphs(c);
+    free_psbb(c);
+    if (!(tau > 0)) return LBM_OK;
+    // per wet node n next to a wet obstacle o = n - c_d: bit d of the mask
+    // and o's weight in plane d
+    const size_t plane = (size_t)c->plane;
+    std::vector<uint16_t> mask(plane, 0);
+    std::vector<double> wb(9 * plane, 0.0);
+    auto cidx = [&](int64_t i, int64_t j) {
+        const size_t w = (size_t)((j + 1) * c->wl + 1 + i / 64);
+        const uint64_t below = (i % 64) ? (c->h_wet[w] & ((1ull << (i % 64)) - 1ull)) : 0ull;
+        return (int64_t)c->h_sb[w] + __builtin_popcountll(below);
+    };
+    for (int64_t q = 0; q < nob; ++q) {
+        const int64_t i = c->obst_ij[2 * q], j = c->obst_ij[2 * q + 1];
+        for (int d = 1; d < 9; ++d)
+            if ((c->obst_mask[q] >> d) & 1) {
+                // the obstacle's bit d names its wet neighbour at the push target o + c_d
+                const int64_t n = cidx(i + kOffI[d], j + kOffJ[d]);
+                mask[(size_t)n] |= (uint16_t)(1u << d);
+                wb[(size_t)d * plane + (size_t)n] = b ? b[q] : 1.0;
+            }
+    }
+    HIP_TRY(hipMalloc(&c->d_ps_mask, plane * sizeof(uint16_t)));
+    HIP_TRY(hipMemcpy(c->d_ps_mask, mask.data(), plane * sizeof(uint16_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&c->d_ps_b, 9 * plane * sizeof(double)));
+    HIP_TRY(hipMemcpy(c->d_ps_b, wb.data(), 9 * plane * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&c->d_ps_val, 9 * plane * sizeof(double)));
+    HIP_TRY(hipMemset(c->d_ps_val, 0, 9 * plane * sizeof(double)));
+    c->psbb = true;
+    c->ps_tau = tau;
+    return LBM_OK;
+}
+
 int lbm_set_geometry(lbm_ctx *c, const uint8_t *cls)
 {
     if (!c || !cls) return fail(LBM_ERR_INVALID, "null argument");
@@ -615,6 +682,7 @@ int lbm_set_geometry(lbm_ctx *c, const uint8_t *cls)
     c->h_wet = std::move(wet);
     HIP_TRY(hipMalloc(&c->d_sb, sb.size() * sizeof(int32_t)));
     HIP_TRY(hipMemcpy(c->d_sb, sb.data(), sb.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    c->h_sb = sb;
     // Wet runs of the run kernel: in every interior column, the wet rows of
     // the segments the perimeter kernel does not take (the first segment
     // when row 0 is wet, the one holding row NR-1 when that row is wet),

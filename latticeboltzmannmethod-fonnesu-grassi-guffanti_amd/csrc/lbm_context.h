@@ -42,6 +42,7 @@ struct lbm_ctx {
     int64_t nr = 0, nc = 0, ldr = 0, plane = 0, wl = 0, E = 0, segs = 0;
     uint64_t *d_wet = nullptr;
     std::vector<uint64_t> h_wet;  // host copy of the wet words (tile tables)
+    std::vector<int32_t> h_sb;    // host copy of the segment bases
     int32_t *d_sb = nullptr;
     int32_t *d_runs = nullptr;  // wet-run table of the run kernel
     int64_t nruns = 0;
@@ -105,6 +106,12 @@ struct lbm_ctx {
     struct lbm_batch *batch = nullptr;
     // 64-row segments per wave of the step kernel (0 = default_segments)
     int32_t segments = 0;
+    // partially saturated bounce-back of the wet obstacles (lbm_set_partial_bounce_back)
+    bool psbb = false;
+    double ps_tau = 0.0;
+    uint16_t *d_ps_mask = nullptr;
+    double *d_ps_b = nullptr;
+    double *d_ps_val = nullptr;
     // captured step loops, by [buffer holding the state][prescribed-value copy]
     hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
 };

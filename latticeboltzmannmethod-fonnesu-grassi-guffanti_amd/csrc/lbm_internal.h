@@ -80,6 +80,17 @@ struct StepArgs {
     // column and segment index.
     const int32_t *perim;
     int64_t nperim;
+    // Partially saturated bounce-back of the wet obstacles (FP64 storage
+    // only; every node then runs through the general path).  For a wet
+    // node n and direction d whose source n - c_d is a wet obstacle, the
+    // population entering n at the next step is computed by n itself at
+    // the end of this step (PSBounceBackPolicy.hpp:177-297) and kept in
+    // ps_val instead of taking n's own opposite slot.
+    int32_t psbb;
+    double ps_t;             // 1 / tau of the policy
+    const uint16_t *ps_mask; // per wet node (compact index): bit d set when n - c_d is a wet obstacle
+    const double *ps_b;      // [9][plane]: weight b of that obstacle
+    double *ps_val;          // [9][plane]: populations entering n through those slots (read, then rewritten)
 };
 
 // Launch shape value selecting the run kernel: one wave per run of up to 64

@@ -485,23 +485,71 @@ __device__ __forceinline__ void zouhe_pressure64(int bt, double *p, double rho)
 
 // Pre-collision populations of wet node (i, j) in FP64, with boundary
 // handling.  Returns the open-boundary policy of the node (-1 = fluid).
+// Partially saturated bounce-back, gather side: slots whose source is a
+// wet obstacle take the value the node computed for them at the end of the
+// previous step (psbb_update) instead of its own opposite slot.
+__device__ __forceinline__ void psbb_gather(const StepArgs &a, int64_t n, double *p)
+{
+    const uint32_t mask = a.ps_mask[n];
+    if (!mask) return;
+#pragma unroll
+    for (int d = 1; d < 9; ++d)
+        if ((mask >> d) & 1u) p[d] = a.ps_val[(int64_t)d * a.g.plane + n];
+}
+
+// PSBounceBackPolicy<2>::update_boundaries (PSBounceBackPolicy.hpp:177-297)
+// for the slots of node n whose source o = n - c_d is a wet obstacle: with
+// q = opp(d), p_q = F*_q(n) (what n streamed into o), P_d the population
+// n held in slot d before this step, e the equilibrium of n at this step
+// (DefaultEquilibrium forms, as the policy restates them) and rho_o = 1
+// (an obstacle's global_rho keeps its initial value, Lattice.hpp:295):
+//   P'_d = p_q - (1 - b) (p_q - e_q) t + b ((P_d - e_d) - (p_q - 2 w_d rho_o))
+// with t = 1 / tau of the policy and 2 w_d written 2 * (1/18) on the axes
+// and 1/36 on the diagonals, in the reference's operation order.
+__device__ __forceinline__ void psbb_update(const StepArgs &a, int64_t n, const double *p, const double *eq,
+                                            const double *o)
+{
+    const uint32_t mask = a.ps_mask[n];
+    if (!mask) return;
+    constexpr double one18 = (1.0 / 18.0);
+    constexpr double one36 = (1.0 / 36.0);
+    const double rho_o = 1.0;
+    const double t = a.ps_t;
+#pragma unroll
+    for (int d = 1; d < 9; ++d) {
+        if (!((mask >> d) & 1u)) continue;
+        const int q = kOpp[d];
+        const int64_t k = (int64_t)d * a.g.plane + n;
+        const double b = a.ps_b[k];
+        const double wr = d < 5 ? 2 * one18 * rho_o : one36 * rho_o;
+        const double pq = o[q];
+        a.ps_val[k] = pq - (1.0 - b) * (pq - eq[q]) * t + b * ((p[d] - eq[d]) - (pq - wr));
+    }
+}
+
 template <bool GATHER>
 __device__ __forceinline__ int load_p64(const StepArgs &a, int i, int j, const Idx &x, uint32_t m, bool edge,
                                         int64_t &e, double *p)
 {
     int policy = -1;
     if (!edge) {
-        if (GATHER) gather_interior(a, x, m, p);
-        else
+        if (GATHER) {
+            gather_interior(a, x, m, p);
+            if (a.psbb) psbb_gather(a, x.src[0], p);
+        } else {
 #pragma unroll
             for (int d = 0; d < 9; ++d) p[d] = ld((const double *)a.fin_plane[d], (uint32_t)x.src[0] * 8u);
+        }
         return -1;
     }
     e = edge_slot(i, j, a.g.nr, a.g.nc);
-    if (GATHER) gather_edge(a, (const double *)a.frozen, i, j, x, e, m, p);
-    else
+    if (GATHER) {
+        gather_edge(a, (const double *)a.frozen, i, j, x, e, m, p);
+        if (a.psbb) psbb_gather(a, x.src[0], p);
+    } else {
 #pragma unroll
         for (int d = 0; d < 9; ++d) p[d] = ld((const double *)a.fin_plane[d], (uint32_t)x.src[0] * 8u);
+    }
     int kind = a.g.edge_kind[e];
     if (kind != kEdgePlain) {
         policy = kind == kEdgeInlet ? a.inlet_policy : a.outlet_policy;
@@ -521,7 +569,7 @@ __device__ __forceinline__ int load_p64(const StepArgs &a, int i, int j, const I
 // (Lattice.hpp:295; the Zou-He density write is lost on a by-value copy,
 // ZouHePolicy.hpp:58,71) and the prescribed u.
 __device__ __forceinline__ void collide_f64(const StepArgs &a, const double *p, int policy, int64_t e, int i, int j,
-                                            double *o)
+                                            double *o, double *eq_out = nullptr)
 {
     double rho, ux, uy;
     if (policy == kVelocity || policy == kNoop) {
@@ -543,6 +591,9 @@ __device__ __forceinline__ void collide_f64(const StepArgs &a, const double *p, 
     }
     double eq[9];
     feq64(rho, ux, uy, eq);
+    if (eq_out)
+#pragma unroll
+        for (int d = 0; d < 9; ++d) eq_out[d] = eq[d];
     if (a.collision == 0) {
         // BGKCollision.hpp:140-153
         const double t_ = a.bgk_t, o_m_t = a.bgk_omt;
@@ -579,6 +630,13 @@ __device__ __forceinline__ void node_general_f64(const StepArgs &a, int i, int j
     double p[9];
     int64_t e = 0;
     int policy = load_p64<GATHER>(a, i, j, x, m, edge, e, p);
+    if (a.psbb) {
+        double o[9], eq[9];
+        collide_f64(a, p, policy, e, i, j, o, eq);
+        psbb_update(a, x.src[0], p, eq, o);
+        store9(a, x.src[0], o);
+        return;
+    }
     collide_store_f64(a, p, policy, e, i, j, x.src[0]);
 }
 
@@ -1269,6 +1327,16 @@ __device__ __forceinline__ StepArgs args_at(const BatchArgs &bt, int s)
 // The perimeter share of a gathering step of a batch: one wave per entry
 // (sample, column, first row) of the perimeter table, which lists every
 // 64-row segment holding a wet perimeter node.
+// A gathering time step with every wet node through the general path, one
+// wave per 64-row segment: the step of lattices with partially saturated
+// bounce-back obstacles (psbb_gather / psbb_update), FP64 storage.
+__global__ __launch_bounds__(kBlockRows) void general_kernel(StepArgs a)
+{
+    const Tile t = tile_of(a);
+    if (!t.valid) return;
+    general_segment<double, true>(a, t.j, t.i0, t.lane);
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlockRows) void batch_perimeter_kernel(BatchArgs bt)
 {
@@ -1757,6 +1825,7 @@ int32_t default_segments(int32_t storage) { return storage == kF16 ? 2 : 1; }
 
 hipError_t launch_step(int32_t storage, bool gather, const StepArgs &a, hipStream_t s, int32_t R)
 {
+    if (a.psbb && gather) return launch(general_kernel, blocks_of(a), s, a);
     if (R <= 0) R = default_segments(storage);
     switch (storage) {
     case kF64: return launch_step_r<double>(gather, R, a, s);

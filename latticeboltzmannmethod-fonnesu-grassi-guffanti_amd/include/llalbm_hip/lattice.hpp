@@ -9,6 +9,7 @@
 
 #include <chrono>
 #include <iostream>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -141,6 +142,7 @@ public:
         };
         if (n_steps > 0) push_values(0, true, [] {});
         lbm_check(lbm_init_equilibrium(ctx), "lbm_init_equilibrium");
+        ParallelizationPolicy::setup_obstacles(ctx, obstacle_nodes);
         saves.reset(lattice_dimensions[0], lattice_dimensions[1]);
         auto t0 = std::chrono::high_resolution_clock::now();
         std::int64_t pending = 0;
@@ -239,6 +241,23 @@ public:
             o << "List of fluid nodes: " << std::endl;
             for (const auto &p : fluid_nodes) o << p.coords[0] << " " << p.coords[1] << std::endl;
         }
+    }
+
+    /// Lattice.hpp:245-278: weights b of the obstacles for partially
+    /// saturated bounce-back, the fraction of obstacle nodes in each 5 x 5
+    /// block.  The reference assigns them to copies of the points held in a
+    /// std::map, so the lattice's obstacle list keeps its weights (b = 1
+    /// unless set through get_obstacle_nodes()[k].set_b); this does the same,
+    /// so results stay those of the reference.  The computed weights are
+    /// returned, in obstacle list order.
+    std::vector<double> compute_obstacle_weight()
+    {
+        std::map<std::pair<Index, Index>, std::size_t> count;
+        for (const auto &o : obstacle_nodes) count[{o.coords[0] / 5, o.coords[1] / 5}]++;
+        std::vector<double> w;
+        w.reserve(obstacle_nodes.size());
+        for (const auto &o : obstacle_nodes) w.push_back((double)count[{o.coords[0] / 5, o.coords[1] / 5}] / 25.0);
+        return w;
     }
 
     /// Lattice.hpp:634-637.

@@ -145,6 +145,41 @@ class HIPBounceBackPolicy;
 template <>
 class HIPBounceBackPolicy<2> : public BoundaryPolicyTag, public HIPTag
 {
+public:
+    static constexpr bool partially_saturated = false;
+};
+
+template <std::size_t dim>
+class HIPPSBounceBackPolicy;
+/// Partially saturated bounce-back as the obstacle policy
+/// (PSBounceBackPolicy<2>, PSBounceBackPolicy.hpp:60-297): walls bounce
+/// back as usual, each wet obstacle mixes bounce-back and the neighbour's
+/// equilibrium with its weight b (ObstaclePoint<2>::b).  The policy's own
+/// relaxation time is set by initialize / allowed_tau, as in the reference.
+/// FP64 storage only.
+template <>
+class HIPPSBounceBackPolicy<2> : public BoundaryPolicyTag, public HIPTag
+{
+public:
+    static inline double tau = 0.0;
+    static constexpr bool partially_saturated = true;
+
+    /// PSBounceBackPolicy.hpp:66-79.
+    static void initialize(const double &tau_, const double &max_speed = 0.2)
+    {
+        tau = tau_;
+        std::cout << "Inizialization done:" << std::endl;
+        std::cout << " Tau: " << tau << std::endl;
+        std::cout << " Max Speed: " << max_speed << std::endl;
+        if (tau <= 0.55) std::cout << " The simulation might have stability issues." << std::endl;
+        std::cout << " Allowed Reynolds Number: " << max_speed * 3.0 / (tau - 0.5) << std::endl;
+    }
+
+    /// PSBounceBackPolicy.hpp:89-98.
+    static void allowed_tau(const double &max_speed, const double &reynolds, const double &speed_of_sound_squared = 1.0 / 3.0)
+    {
+        tau = 0.5 + max_speed / (speed_of_sound_squared * reynolds);
+    }
 };
 
 template <std::size_t dim>
@@ -277,20 +312,39 @@ public:
     using CollisionPolicy = typename Configuration::collision_policy_t;
     using InletPolicy = typename Configuration::inlet_policy_t;
     using OutletPolicy = typename Configuration::outlet_policy_t;
+    using ObstaclePolicy = typename Configuration::obstacle_policy_t;
     using InitializationPolicy = typename Configuration::initialization_policy_t;
     using Tensor3 = util::HostTensor<double, 3>;
     using Tensor2 = util::HostTensor<double, 2>;
 
-    /// The context the statics act on, and whether its run has started.
+    /// The context the statics act on, its obstacle list, and whether its
+    /// run has started.
     static inline lbm_ctx *context = nullptr;
+    static inline const std::vector<ObstaclePoint<2>> *obstacles = nullptr;
     static inline bool started = false;
     /// lbm_step flags of the launches collide makes.
     static inline uint32_t step_flags = 0;
 
-    static void bind(lbm_ctx *ctx)
+    static void bind(lbm_ctx *ctx, const std::vector<ObstaclePoint<2>> *obstacle_list = nullptr)
     {
         context = ctx;
+        obstacles = obstacle_list;
         started = false;
+    }
+
+    /// The obstacle policy's set-up at the start of a run: partially
+    /// saturated bounce-back hands its tau and the obstacles' weights (list
+    /// order) to the context; plain bounce-back needs nothing.
+    static void setup_obstacles(lbm_ctx *ctx, const std::vector<ObstaclePoint<2>> &list)
+    {
+        if constexpr (requires { ObstaclePolicy::partially_saturated; }) {
+            if constexpr (ObstaclePolicy::partially_saturated) {
+                std::vector<double> b(list.size());
+                for (std::size_t k = 0; k < list.size(); ++k) b[k] = list[k].get_b();
+                check(lbm_set_partial_bounce_back(ctx, ObstaclePolicy::tau, b.data()),
+                      "lbm_set_partial_bounce_back");
+            }
+        }
     }
 
     // ----- update of the boundaries (OMPPolicy.hpp:50-72): in the fused step
@@ -390,6 +444,7 @@ private:
         require_context();
         if (started) return;
         check(lbm_init_equilibrium(context), "lbm_init_equilibrium");
+        if (obstacles) setup_obstacles(context, *obstacles);
         started = true;
     }
 };

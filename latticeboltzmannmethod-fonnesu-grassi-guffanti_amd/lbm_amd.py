@@ -35,7 +35,8 @@ ERR_NAMES = {0: "OK", 1: "INVALID", 2: "HIP", 3: "NAN", 4: "STATE", 5: "UNSUPPOR
 ERR_INVALID, ERR_STATE, ERR_NAN, ERR_UNSUPPORTED, ERR_RANGE = 1, 4, 3, 5, 7
 
 EXPORTED = [
-    "lbm_version", "lbm_last_error", "lbm_create", "lbm_destroy", "lbm_set_relaxation", "lbm_set_geometry",
+    "lbm_version", "lbm_last_error", "lbm_create", "lbm_destroy", "lbm_set_relaxation",
+    "lbm_set_partial_bounce_back", "lbm_set_geometry",
     "lbm_node_counts", "lbm_open_nodes", "lbm_set_open_values", "lbm_init_equilibrium",
     "lbm_upload_populations", "lbm_download_populations", "lbm_download_after_collision", "lbm_step",
     "lbm_get_time", "lbm_download_macro", "lbm_download_macro_async", "lbm_wait", "lbm_host_alloc",
@@ -88,6 +89,7 @@ def lib():
             "lbm_destroy": [vp],
             "lbm_set_relaxation": [vp, C.c_double, C.c_double, C.c_double],
             "lbm_set_geometry": [vp, vp],
+            "lbm_set_partial_bounce_back": [vp, C.c_double, dp],
             "lbm_node_counts": [vp, i64p],
             "lbm_open_nodes": [vp, C.c_int, i64p, i64p],
             "lbm_set_open_values": [vp, C.c_int, dp, dp, dp],
@@ -233,6 +235,13 @@ class Lattice:
 
     def set_relaxation(self, tau=1.0, tau_even=1.0, tau_odd=1.0):
         _check(lib().lbm_set_relaxation(self._h, tau, tau_even, tau_odd))
+
+    def set_partial_bounce_back(self, tau, b=None):
+        """Partially saturated bounce-back on the wet obstacles (FP64):
+        the policy's tau and one weight per wet obstacle in list order
+        (None = 1 for all); tau <= 0 switches it off."""
+        arr = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+        _check(lib().lbm_set_partial_bounce_back(self._h, float(tau), None if arr is None else _dp(arr)))
 
     def node_counts(self):
         out = np.zeros(7, dtype=np.int64)

@@ -27,6 +27,8 @@
  *   node identification  llalbm/src/utils/analysis/NodeIdentification.hpp:45-242
  *   list construction    llalbm/src/utils/readers/LatticeReader.hpp:136-205 (row-major scan)
  *   lift / drag          llalbm/src/utils/analysis/FlowAnalyzer.hpp:348-451
+ *   partially saturated  llalbm/src/core/boundaries/PSBounceBackPolicy.hpp:177-297
+ *   bounce-back          (weights: Lattice.hpp:245-278, ObstaclePoint b, aliases.hpp:340-357)
  *
  * Extensions with no reference counterpart (defined here, identically in
  * the HIP path):
@@ -69,6 +71,8 @@ typedef struct orc_lattice {
     double *val_ux[2], *val_uy[2], *val_rho[2];
     int64_t step;
     int by_value;             /* timing baseline: make the reference's by-value field copies */
+    double ps_tau;            /* > 0: obstacles are partially saturated bounce-back nodes */
+    double *ps_b;             /* their weights, obstacle list order */
 } orc_lattice;
 
 #define P3(L, a, i, j, d) ((a)[(i) + (L)->nr * ((j) + (L)->nc * (int64_t)(d))])
@@ -127,6 +131,7 @@ void orc_destroy(orc_lattice *L)
     free(L->P); free(L->E); free(L->A); free(L->rho); free(L->u);
     for (int k = 0; k < 5; ++k) free(L->list[k]);
     for (int k = 0; k < 2; ++k) { free(L->val_ux[k]); free(L->val_uy[k]); free(L->val_rho[k]); }
+    free(L->ps_b);
     free(L);
 }
 
@@ -477,6 +482,55 @@ static void bounce_back_obstacles(orc_lattice *L)
     }
 }
 
+/* PSBounceBackPolicy<2>::update_boundaries (PSBounceBackPolicy.hpp:177-297),
+ * in list order: obstacle o reads its populations p1..p8 (pushed into it by
+ * its wet neighbours), then for every direction bit d writes slot d of the
+ * neighbour n = o + c_d from p_opp(d), the equilibrium of n at this step
+ * (global_rho / global_u of n, DefaultEquilibrium's forms) and the value
+ * slot d of n held before, with weight b of o and t = 1 / tau of the
+ * policy.  global_rho of an obstacle keeps its initial 1. */
+static void partially_saturated_obstacles(orc_lattice *L)
+{
+    const double one18 = (1.0 / 18.0);
+    const double one36 = (1.0 / 36.0);
+    const node_t *lst = L->list[LIST_OBSTACLE];
+    for (int64_t q = 0; q < L->count[LIST_OBSTACLE]; ++q) {
+        const int64_t i = lst[q].i, j = lst[q].j;
+        const int32_t mask = lst[q].tag;
+        const double b = L->ps_b[q];
+        double p[9];
+        for (int d = 1; d < 9; ++d) p[d] = P3(L, L->P, i, j, d);
+        const double t = 1.0 / L->ps_tau;
+        const double rho_o = R2(L, L->rho, i, j);
+        for (int d = 1; d < 9; ++d) {
+            if (!(mask & (1 << d))) continue;
+            const int64_t ni = i + OFF_I[d], nj = j + OFF_J[d];
+            double e[9];
+            orc_feq(R2(L, L->rho, ni, nj), L->u[ni + L->nr * nj], L->u[ni + L->nr * (nj + L->nc)], e);
+            static const int OPP[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+            const int o = OPP[d];
+            const double po = p[o];
+            const double old = P3(L, L->P, ni, nj, d);
+            if (d < 5)
+                P3(L, L->P, ni, nj, d) = po - (1.0 - b) * (po - e[o]) * t + b * ((old - e[d]) - (po - 2 * one18 * rho_o));
+            else
+                P3(L, L->P, ni, nj, d) = po - (1.0 - b) * (po - e[o]) * t + b * ((old - e[d]) - (po - one36 * rho_o));
+        }
+    }
+}
+
+void orc_set_partial_bounce_back(orc_lattice *L, double tau, const double *b)
+{
+    free(L->ps_b);
+    L->ps_b = NULL;
+    L->ps_tau = 0.0;
+    if (!(tau > 0)) return;
+    int64_t n = L->count[LIST_OBSTACLE];
+    L->ps_b = (double *)malloc(sizeof(double) * (size_t)(n ? n : 1));
+    for (int64_t q = 0; q < n; ++q) L->ps_b[q] = b ? b[q] : 1.0;
+    L->ps_tau = tau;
+}
+
 /* ZouHePolicy.hpp:58-123 (velocity), pressure extension, or no-op
  * (OpenBoundaryPolicy.hpp:58-61) for one open list. */
 static void update_open(orc_lattice *L, int k)
@@ -563,7 +617,8 @@ static void one_step(orc_lattice *L)
     if (L->by_value) by_value_fields(L, 1); /* update_domain_boundaries */
     bounce_back_walls(L);
     if (L->by_value) by_value_fields(L, 1); /* update_obstacle_boundaries */
-    bounce_back_obstacles(L);
+    if (L->ps_tau > 0) partially_saturated_obstacles(L);
+    else bounce_back_obstacles(L);
     if (L->by_value) by_value_fields(L, 1); /* update_inlet_boundaries */
     update_open(L, 0);
     if (L->by_value) by_value_fields(L, 1); /* update_outlet_boundaries */

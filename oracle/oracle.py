@@ -70,6 +70,7 @@ def lib():
         L.orc_lift_drag.argtypes = [C.c_void_p, i64p, i32p, C.c_int64, dp, dp]
         L.orc_reduce.argtypes = [C.c_void_p, dp]
         L.orc_set_by_value_copies.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_partial_bounce_back.argtypes = [C.c_void_p, C.c_double, dp]
         _lib = L
     return _lib
 
@@ -221,6 +222,12 @@ class OracleLattice:
         if per_point:
             return ld[0], ld[1], pp[:n]
         return ld[0], ld[1]
+
+    def set_partial_bounce_back(self, tau, b=None):
+        """PSBounceBackPolicy<2> on the obstacle list: the policy's tau and one
+        weight per wet obstacle in list order (None = 1); tau <= 0 = off."""
+        arr = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+        lib().orc_set_partial_bounce_back(self._h, float(tau), None if arr is None else _dp(arr))
 
     def set_by_value_copies(self, on=True):
         """CPU timing baseline only: make the reference's by-value copies of

@@ -61,7 +61,7 @@ int main(int argc, char **argv)
     auto &out = lattice.get_outlet_nodes();
     auto &obst = lattice.get_obstacle_nodes();
 
-    P::bind(lattice.context());
+    P::bind(lattice.context(), &obst);
     rho.setConstant(1);
     u.setConstant(0);
     P::calc_equilibrium(fluid, populations, u, rho);

@@ -306,3 +306,37 @@ def test_policy_statics_reference_loop(tmp_path):
     y10 = 10 - 0.5
     want = 4.0 * 0.05 * y10 * (46.0 - y10) / (46.0 * 46.0) * (1.0 - math.exp(-(steps - 1) * 0.01))
     assert f"inlet u(10, 0) {want:.17g}" in out
+
+
+@pytest.mark.gpu
+def test_ps_obstacle_driver_matches_oracle(tmp_path):
+    """test/unit/LidWithPSObstacle.cpp through the compat layer
+    (HIPPSBounceBackPolicy as the obstacle policy, its tau from
+    allowed_tau(0.02, 10), compute_obstacle_weight leaving b = 1 as the
+    reference does) against the oracle's PSBounceBackPolicy over 400
+    steps, to 1e-12."""
+    steps = 400
+    out = _run("ps_obstacle", [steps, tmp_path / "pop.bin"], tmp_path)
+    cls = L.geometry_cavity(100, 100)
+    L.add_disc(cls, 50, 50, 20)
+    te = 0.9
+    to = 0.25 * (1.0 / (te - 0.5)) + 0.5
+    tau_ps = 0.5 + 0.02 / ((1.0 / 3.0) * 10)
+    orc = O.OracleLattice(cls, collision=O.TRT, tau_even=te, tau_odd=to, inlet_policy=O.ZOUHE_VELOCITY,
+                          outlet_policy=O.ZOUHE_VELOCITY)
+    tok = out.split()
+    assert int(tok[tok.index("obstacles") + 1]) == orc.count(O.LIST_OBSTACLE)
+    assert float(tok[tok.index("tau_ps") + 1]) == tau_ps
+    assert float(tok[tok.index("b0") + 1]) == 1.0          # the list keeps its weights
+    assert float(tok[tok.index("min_block_weight") + 1]) < 1.0   # though partial blocks were computed
+    orc.set_partial_bounce_back(tau_ps)
+    ni = orc.count(O.LIST_INLET)
+    orc.init()
+    for i in range(steps):
+        orc.set_open_values(0, ux=np.full(ni, 0.2 * (1.0 - math.exp(-((500 * 500 * i) / (2 * 1000 * 1000))))),
+                            uy=np.zeros(ni))
+        orc.step(1)
+    pop = np.fromfile(tmp_path / "pop.bin").reshape(9, 100, 100)
+    err = max_abs_wet(pop, orc.populations(), cls)
+    print(f"PS obstacle driver vs oracle: max|dP| = {err:.3e}", flush=True)
+    assert err <= 1e-12

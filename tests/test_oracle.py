@@ -479,3 +479,113 @@ def test_reduce_matches_population_sums():
     assert np.isclose(out[3], pf.sum(), rtol=1e-14, atol=0)
     assert out[4] == fl.sum()
     assert abs(out[1]) > 1e-3  # the flow carries momentum: the check has teeth
+
+
+# ------------------------------------------- partially saturated bounce-back
+
+def _ps_lattice(b=None, tau_ps=None, p=None):
+    """Lid cavity 22 x 26 with two obstacles (a disc and a block), BGK;
+    PSBounceBackPolicy on the obstacles when tau_ps is given."""
+    nr, nc = 22, 26
+    cls = _box(nr, nc)
+    cls[1:-1, 0] = O.INLET
+    ii, jj = np.meshgrid(np.arange(nr), np.arange(nc))
+    cls[(ii - 9) ** 2 + (jj - 8) ** 2 <= 9] = O.OBSTACLE
+    cls[16:19, 13:17] = O.OBSTACLE
+    lat = O.OracleLattice(cls, collision=O.BGK, tau=0.8, inlet_policy=O.ZOUHE_VELOCITY, outlet_policy=O.NOOP)
+    ni = lat.count(O.LIST_INLET)
+    lat.set_open_values(0, ux=np.full(ni, 0.06), uy=np.zeros(ni))
+    if tau_ps is not None:
+        lat.set_partial_bounce_back(tau_ps, b)
+    lat.init()
+    if p is not None:
+        lat.set_populations(p)
+    return cls, lat
+
+
+def _ps_slots(lat):
+    """(row, column, direction, obstacle index) of every slot a wet obstacle
+    writes: direction d of the neighbour o + c_d for each bit d."""
+    ij, masks = lat.node_list(O.LIST_OBSTACLE)
+    out = []
+    for q, ((i, j), m) in enumerate(zip(ij, masks)):
+        for d in range(1, 9):
+            if m & (1 << d):
+                out.append((i + DROW[d], j + DCOL[d], d, q))
+    return out
+
+
+def _state_after(steps):
+    cls, lat = _ps_lattice(tau_ps=0.9)
+    lat.step(steps)
+    return cls, lat.populations()
+
+
+def _moments_feq(p, i, j):
+    f = p[:, j, i]
+    rho = f.sum()
+    ux, uy = (CX * f).sum() / rho, (CY * f).sum() / rho
+    return _feq_textbook(rho, np.array([ux]), np.array([uy]))[:, 0]
+
+
+def test_psbb_weight_zero_with_infinite_tau_is_bounce_back():
+    """b = 0 and tau -> infinity leave p_q - (p_q - e_q) / tau = p_q: the
+    policy reduces to half-way bounce-back, bit for bit, over 60 steps."""
+    cls, plain = _ps_lattice()
+    _, ps = _ps_lattice(b=np.zeros(plain.count(O.LIST_OBSTACLE)), tau_ps=1e300)
+    plain.step(60)
+    ps.step(60)
+    wet = wet_mask(cls)
+    assert np.array_equal(plain.populations()[:, wet], ps.populations()[:, wet])
+
+
+def test_psbb_closed_forms_of_one_step():
+    """From the same state, one step with weight b = 1 gives, on every slot an
+    obstacle writes, P_d(n) - e_d(n) + w_d (the old slot value relaxed by
+    the equilibrium of n, the obstacle's rho = 1); with b = 0 and tau = 1 it
+    gives e_opp(d)(n); with b = 0.3 the weighted mix of both.  Equilibria
+    from the textbook form at the moments of the state."""
+    cls, p0 = _state_after(40)
+    _, lat = _ps_lattice(tau_ps=0.9)
+    n_obst = lat.count(O.LIST_OBSTACLE)
+    slots = _ps_slots(lat)
+    assert len(slots) > 20
+    res = {}
+    for name, b, tau in (("b1", np.ones(n_obst), 0.9), ("b0", np.zeros(n_obst), 1.0),
+                         ("mix", np.full(n_obst, 0.3), 1.0)):
+        _, l2 = _ps_lattice(b=b, tau_ps=tau, p=p0)
+        l2.step(1)
+        res[name] = l2.populations()
+    for (i, j, d, q) in slots:
+        e = _moments_feq(p0, i, j)
+        want1 = p0[d, j, i] - e[d] + W[d]
+        assert abs(res["b1"][d, j, i] - want1) <= 1e-15, (i, j, d)
+        assert abs(res["b0"][d, j, i] - e[OPP[d]]) <= 1e-15, (i, j, d)
+        mix = 0.7 * res["b0"][d, j, i] + 0.3 * res["b1"][d, j, i]
+        assert abs(res["mix"][d, j, i] - mix) <= 1e-15, (i, j, d)
+    # the other wet slots do not depend on the weights
+    wet = wet_mask(cls)
+    other = np.ones_like(res["b1"], dtype=bool)
+    for (i, j, d, q) in slots:
+        other[d, j, i] = False
+    other &= wet[None]
+    assert np.array_equal(res["b0"][other], res["mix"][other])
+
+
+def test_psbb_uses_each_obstacles_own_weight():
+    """Weights differ per obstacle (list order): every written slot is the
+    mix, with the weight of the obstacle it comes from, of the b = 0 and
+    b = 1 results."""
+    cls, p0 = _state_after(25)
+    _, lat = _ps_lattice(tau_ps=1.3)
+    n_obst = lat.count(O.LIST_OBSTACLE)
+    rng = np.random.default_rng(2)
+    bw = rng.uniform(0, 1, n_obst)
+    out = {}
+    for name, b in (("0", np.zeros(n_obst)), ("1", np.ones(n_obst)), ("w", bw)):
+        _, l2 = _ps_lattice(b=b, tau_ps=1.3, p=p0)
+        l2.step(1)
+        out[name] = l2.populations()
+    for (i, j, d, q) in _ps_slots(lat):
+        mix = (1 - bw[q]) * out["0"][d, j, i] + bw[q] * out["1"][d, j, i]
+        assert abs(out["w"][d, j, i] - mix) <= 2e-15, (i, j, d, q)
