@@ -323,12 +323,16 @@ int lbm_batch_destroy(lbm_batch *batch);
 /* Tile shape: `segments` 64-row segments per wave (1, 2, 4 or 8), and
  * either a persistent grid of `blocks` work-groups of 256 threads that
  * walks the tile table (rounded up to a multiple of 8, one share per XCD)
- * or, with blocks = 0 (the default), one wave per tile. */
+ * or, with blocks = 0 (the default), one wave per tile.
+ * segments = LBM_LAUNCH_WET_RUNS (blocks = 0) takes one wave per run of up
+ * to 64 consecutive wet rows of a sample's column, over every sample's run
+ * table (as lbm_set_launch for one lattice). */
 int lbm_batch_set_launch(lbm_batch *batch, int32_t segments, int64_t blocks);
 int lbm_batch_get_launch(lbm_batch *batch, int32_t *segments, int64_t *blocks);
 
 /* Autotuner: times every candidate tile shape and grid (one wave per tile,
- * or 2, 4, 8, 16 persistent work-groups per compute unit) over `steps`
+ * or 2, 4, 8, 16 persistent work-groups per compute unit; and wet-run
+ * waves) over `steps`
  * launches each (a launch rewrites the spare buffer from the current
  * state, so the state does not change) and keeps the fastest.  Reports
  * the choice and its time per step (any output may be NULL). */

@@ -58,6 +58,9 @@ struct lbm_batch {
     // segments holding a wet perimeter node (the general path's share)
     int32_t *d_ptiles = nullptr;
     int64_t nptiles = 0;
+    // wet-run table (launch shape kRunLaunch): every sample's runs
+    int32_t *d_runs = nullptr;
+    int64_t nruns = 0;
     // "first" = each sample's arguments as lbm_step would build them for its
     // next step; "steady" = the same with the previous prescribed values set
     // to the current ones (every step after the first of a run)
@@ -192,6 +195,24 @@ int build_perimeter_tiles(lbm_batch *b)
     return LBM_OK;
 }
 
+// Wet-run table of the ensemble: each sample's own run table (built with
+// its geometry) behind the sample index, in sample order.
+int build_runs(lbm_batch *b)
+{
+    std::vector<int32_t> runs;
+    for (size_t s = 0; s < b->samples.size(); ++s) {
+        const std::vector<int32_t> &r = b->samples[s]->h_runs;
+        for (size_t k = 0; k + 3 < r.size(); k += 4) {
+            runs.push_back((int32_t)s);
+            runs.insert(runs.end(), r.begin() + (long)k, r.begin() + (long)k + 4);
+        }
+    }
+    b->nruns = (int64_t)(runs.size() / 5);
+    HIP_TRY(hipMalloc(&b->d_runs, std::max<size_t>(runs.size(), 5) * sizeof(int32_t)));
+    HIP_TRY(hipMemcpy(b->d_runs, runs.data(), runs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    return LBM_OK;
+}
+
 int64_t round8(int64_t v) { return std::max<int64_t>(8, (v + 7) / 8 * 8); }
 
 // Brings the device arguments up to date with the samples: a sample whose
@@ -237,6 +258,8 @@ BatchArgs batch_args(const lbm_batch *b, const ArgBuffer &ab, int parity)
     a.nptiles = b->nptiles;
     a.parity = parity;
     a.direct = 0;
+    a.runs = b->d_runs;
+    a.nruns = b->nruns;
     return a;
 }
 
@@ -318,6 +341,7 @@ int lbm_batch_create(lbm_ctx *const *samples, int64_t n, lbm_batch **out)
     if (!r) r = alloc_args(b->steady, (size_t)(2 * n));
     if (!r) r = build_tiles(b, b->R);
     if (!r) r = build_perimeter_tiles(b);
+    if (!r) r = build_runs(b);
     if (r) return bail(r);
     for (lbm_ctx *c : b->samples) c->batch = b;
     *out = b;
@@ -336,6 +360,7 @@ int lbm_batch_destroy(lbm_batch *b)
     free_args(b->steady);
     (void)hipFree(b->d_tiles);
     (void)hipFree(b->d_ptiles);
+    (void)hipFree(b->d_runs);
     (void)hipFree(b->d_tile_begin);
     (void)hipFree(b->d_flags);
     (void)hipFree(b->d_partials);
@@ -350,13 +375,18 @@ int lbm_batch_destroy(lbm_batch *b)
 int lbm_batch_set_launch(lbm_batch *b, int32_t segments, int64_t blocks)
 {
     if (!b) return fail(LBM_ERR_INVALID, "null batch");
-    if (segments != 1 && segments != 2 && segments != 4 && segments != 8)
-        return fail(LBM_ERR_INVALID, "segments must be 1, 2, 4 or 8");
+    if (segments != 1 && segments != 2 && segments != 4 && segments != 8 && segments != kRunLaunch)
+        return fail(LBM_ERR_INVALID, "segments must be 1, 2, 4, 8 or %d (wet runs)", kRunLaunch);
     if (blocks < 0 || blocks > (1 << 24)) return fail(LBM_ERR_INVALID, "grid size out of range");
+    if (segments == kRunLaunch && blocks != 0)
+        return fail(LBM_ERR_INVALID, "wet-run waves take one wave per run (blocks = 0)");
     HIP_TRY(hipSetDevice(b->device));
     const int64_t nb = blocks == 0 ? 0 : round8(blocks);
-    if (segments != b->tiles_R) {
-        int r = build_tiles(b, segments);
+    // wet runs keep a one-segment tile table for the collide-only first
+    // step and the reductions
+    const int32_t tiles_R = segments == kRunLaunch ? 1 : segments;
+    if (tiles_R != b->tiles_R) {
+        int r = build_tiles(b, tiles_R);
         if (r) return r;
     }
     if (nb != b->blocks || segments != b->R) drop_graphs(b);
@@ -382,15 +412,16 @@ int lbm_batch_autotune(lbm_batch *b, int32_t steps, int32_t *segments, int64_t *
     bool changed = false;
     int r = refresh(b, changed);
     if (r) return r;
-    const int32_t Rs[] = {1, 2, 4, 8};
+    const int32_t Rs[] = {1, 2, 4, 8, kRunLaunch};
     const int64_t per_cu[] = {0, 2, 4, 8, 16};  // 0: one wave per tile
     float best = -1.f;
     int32_t best_R = b->R;
     int64_t best_nb = b->blocks;
     for (int32_t R : Rs) {
-        r = build_tiles(b, R);
+        r = build_tiles(b, R == kRunLaunch ? 1 : R);
         if (r) return r;
         for (int64_t k : per_cu) {
+            if (R == kRunLaunch && k != 0) continue;  // wet runs: one wave per run only
             const int64_t nb = k == 0 ? 0 : round8(k * (int64_t)b->cus);
             // every launch recomputes the spare buffer from the current
             // state (fixed parity), so the state is unchanged
@@ -409,7 +440,7 @@ int lbm_batch_autotune(lbm_batch *b, int32_t steps, int32_t *segments, int64_t *
             }
         }
     }
-    r = build_tiles(b, best_R);
+    r = build_tiles(b, best_R == kRunLaunch ? 1 : best_R);
     if (r) return r;
     b->R = best_R;
     b->blocks = best_nb;

@@ -724,6 +724,7 @@ int lbm_set_geometry(lbm_ctx *c, const uint8_t *cls)
         c->nruns = (int64_t)(runs.size() / 4);
         HIP_TRY(hipMalloc(&c->d_runs, std::max<size_t>(runs.size(), 4) * sizeof(int32_t)));
         HIP_TRY(hipMemcpy(c->d_runs, runs.data(), runs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        c->h_runs = std::move(runs);
     }
     // Perimeter items: the 64-row segments holding a wet perimeter node,
     // which the step kernels skip (every wet segment of columns 0 and NC-1;

@@ -110,6 +110,11 @@ struct BatchArgs {
     int64_t nptiles;
     int32_t parity;
     int32_t direct;  // set by launch_batch_step: one wave per tile instead of a persistent grid
+    // wet-run table of the ensemble (launch shape kRunLaunch): per run, 5
+    // int32 -- sample, column, first row, compact index of the first row,
+    // wet rows (each sample's own run table, as run_kernel reads it)
+    const int32_t *runs;
+    int64_t nruns;
 };
 
 // Per-sample FP64 sums (readout's five) of a batch's current state:
@@ -127,7 +132,9 @@ int32_t default_segments(int32_t storage);
 hipError_t launch_step(int32_t storage, bool gather, const StepArgs &a, hipStream_t s, int32_t R = 0);
 // One time step of every lattice of a batch: on a persistent grid of
 // `blocks` work-groups (rounded up to a multiple of 8), or with one wave per
-// tile when blocks <= 0.
+// tile when blocks <= 0; R = kRunLaunch: one wave per entry of the batch's
+// wet-run table (the collide-only first step then takes the tiles, built
+// for R = 1).
 hipError_t launch_batch_step(int32_t storage, bool gather, int32_t R, const BatchArgs &b, int64_t blocks,
                              hipStream_t s);
 // P (or F* when gather is false) of every node in the reference layout

@@ -1151,14 +1151,9 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
 // past the run read their first row's slot (bounce-back addresses, in
 // range) and do not store.  Loads come before any branch (see step_wave).
 template <typename T>
-__device__ __forceinline__ void run_wave(const StepArgs &a, int64_t t, int lane, int32_t *rows)
+__device__ __forceinline__ void run_wave(const StepArgs &a, int j, int i0, int s0, int n, int lane, int32_t *rows)
 {
     using V = typename ComputeOf<T>::type;
-    const LBM_CONST int32_t *tp = (const LBM_CONST int32_t *)a.runs + 4 * t;
-    const int j = __builtin_amdgcn_readfirstlane(tp[0]);
-    const int i0 = __builtin_amdgcn_readfirstlane(tp[1]);
-    const int s0 = __builtin_amdgcn_readfirstlane(tp[2]);
-    const int n = __builtin_amdgcn_readfirstlane(tp[3]);
     const int w0 = i0 >> 6;
     // words w0-1 .. w0+3 of columns j-1, j, j+1 and the segment bases of
     // words w0 .. w0+2.  A column keeps three zero guard words past its last
@@ -1282,7 +1277,10 @@ __global__ __launch_bounds__(kBlockRows) void run_kernel(StepArgs a)
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t t = b * (kBlockRows / 64) + wave;
     if (t >= a.nruns) return;
-    run_wave<T>(a, t, threadIdx.x & 63, rows[wave]);
+    const LBM_CONST int32_t *tp = (const LBM_CONST int32_t *)a.runs + 4 * t;
+    run_wave<T>(a, __builtin_amdgcn_readfirstlane(tp[0]), __builtin_amdgcn_readfirstlane(tp[1]),
+                __builtin_amdgcn_readfirstlane(tp[2]), __builtin_amdgcn_readfirstlane(tp[3]), threadIdx.x & 63,
+                rows[wave]);
 }
 
 // The perimeter share of a gathering step of one lattice: one wave per
@@ -1335,6 +1333,29 @@ __global__ __launch_bounds__(kBlockRows) void general_kernel(StepArgs a)
     const Tile t = tile_of(a);
     if (!t.valid) return;
     general_segment<double, true>(a, t.j, t.i0, t.lane);
+}
+
+// One time step of every lattice of an ensemble in wet-run waves: one wave
+// per entry of the ensemble's run table (sample, then the sample's own run
+// entry), each XCD taking one contiguous part of the table.  The samples'
+// perimeter segments run in batch_perimeter_kernel, as for the tiles.
+template <typename T>
+__global__ __launch_bounds__(kBlockRows) void batch_run_kernel(BatchArgs bt)
+{
+    __shared__ int32_t rows[kBlockRows / 64][64];
+    asm volatile("" ::"s"(bt.nruns), "s"(bt.runs), "s"(bt.args), "s"(bt.parity));  // one round trip
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t nb = (uint32_t)((bt.nruns + kBlockRows / 64 - 1) / (kBlockRows / 64));
+    const int64_t t = (int64_t)xcd_remap(blockIdx.x, nb) * (kBlockRows / 64) + wave;
+    if (t >= bt.nruns) return;
+    const LBM_CONST int32_t *e = (const LBM_CONST int32_t *)bt.runs + 5 * t;
+    const int s = __builtin_amdgcn_readfirstlane(e[0]);
+    const int j = __builtin_amdgcn_readfirstlane(e[1]);
+    const int i0 = __builtin_amdgcn_readfirstlane(e[2]);
+    const int s0 = __builtin_amdgcn_readfirstlane(e[3]);
+    const int n = __builtin_amdgcn_readfirstlane(e[4]);
+    const StepArgs a = args_at(bt, s);
+    run_wave<T>(a, j, i0, s0, n, threadIdx.x & 63, rows[wave]);
 }
 
 template <typename T>
@@ -1764,6 +1785,19 @@ hipError_t launch_batch_t(bool gather, const BatchArgs &b, int64_t blocks, hipSt
 template <typename T>
 hipError_t launch_batch_r(bool gather, int32_t R, const BatchArgs &b, int64_t blocks, hipStream_t s)
 {
+    if (R == kRunLaunch) {
+        if (!gather) return launch_batch_t<T, 1>(false, b, blocks, s);
+        if (b.nruns > 0) {
+            const int64_t nb = (b.nruns + kBlockRows / 64 - 1) / (kBlockRows / 64);
+            hipLaunchKernelGGL((batch_run_kernel<T>), dim3((unsigned)nb), dim3(kBlockRows), 0, s, b);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
+        if (b.nptiles == 0) return hipSuccess;
+        const int64_t pb = (b.nptiles + kBlockRows / 64 - 1) / (kBlockRows / 64);
+        hipLaunchKernelGGL((batch_perimeter_kernel<T>), dim3((unsigned)pb), dim3(kBlockRows), 0, s, b);
+        return hipGetLastError();
+    }
     switch (R) {
     case 1: return launch_batch_t<T, 1>(gather, b, blocks, s);
     case 2: return launch_batch_t<T, 2>(gather, b, blocks, s);
@@ -1839,6 +1873,7 @@ hipError_t launch_batch_step(int32_t storage, bool gather, int32_t R, const Batc
 {
     if (R <= 0) R = default_segments(storage);
     BatchArgs b = b0;
+    if (R == kRunLaunch) blocks = 0;  // one wave per run; the first step takes one wave per tile
     b.direct = blocks <= 0;
     if (b.direct) blocks = std::max<int64_t>(1, (b.ntiles + kBlockRows / 64 - 1) / (kBlockRows / 64));
     else blocks = ((blocks + 7) / 8) * 8;  // whole rounds of the 8 XCDs

@@ -39,13 +39,17 @@ def run_alone(wl, storage, schedule):
     return p
 
 
+@pytest.mark.parametrize("launch", ["tiles", "wet-runs"])
 @pytest.mark.parametrize("storage", [L.F64, L.F32_SHIFTED, L.F16_SHIFTED])
-def test_ensemble_equals_lattices_alone(storage):
+def test_ensemble_equals_lattices_alone(storage, launch):
     wls = small_ensemble()
-    say(f"case ensemble of {len(wls)} ({', '.join(w.name for w in wls)}), storage {storage}")
+    say(f"case ensemble of {len(wls)} ({', '.join(w.name for w in wls)}), storage {storage}, {launch}")
     lats = [w.make_lattice(storage) for w in wls]
     schedule = [1, 40, 7, 64]          # first step, graph + eager remainder, eager, graph replays
     with L.Batch(lats) as b:
+        if launch == "wet-runs":
+            b.set_launch(L.LAUNCH_WET_RUNS, 0)
+            assert b.launch() == (L.LAUNCH_WET_RUNS, 0)
         for k in schedule:
             b.step(k, L.STEP_CHECK)
         t = [lat.time for lat in lats]
@@ -74,12 +78,15 @@ def test_every_tile_shape_gives_the_same_result(segments):
         lat.close()
 
 
-def test_graph_replays_equal_eager_launches():
+@pytest.mark.parametrize("launch", ["tiles", "wet-runs"])
+def test_graph_replays_equal_eager_launches(launch):
     wls = small_ensemble(n=5, seed=23)
-    say("case graph vs eager, 4 calls of 64 steps with reductions and downloads in between")
+    say(f"case graph vs eager ({launch}), 4 calls of 64 steps with reductions and downloads in between")
     a = [w.make_lattice(L.F32_SHIFTED) for w in wls]
     e = [w.make_lattice(L.F32_SHIFTED) for w in wls]
     with L.Batch(a) as ba, L.Batch(e) as be:
+        if launch == "wet-runs":
+            ba.set_launch(L.LAUNCH_WET_RUNS, 0)
         for call in range(4):
             ba.step(64)
             be.step(64, L.STEP_NO_GRAPH)
@@ -91,12 +98,15 @@ def test_graph_replays_equal_eager_launches():
         lat.close()
 
 
-def test_fp64_ensemble_matches_oracle():
+@pytest.mark.parametrize("launch", ["tiles", "wet-runs"])
+def test_fp64_ensemble_matches_oracle(launch):
     wls = W.ensemble(n_samples=3, seed=29, sizes=(128, 192))
     steps = 300
-    say(f"case FP64 ensemble vs oracle (populations and permeability), {steps} steps")
+    say(f"case FP64 ensemble vs oracle (populations and permeability), {steps} steps, {launch}")
     lats = [w.make_lattice(L.F64) for w in wls]
     with L.Batch(lats) as b:
+        if launch == "wet-runs":
+            b.set_launch(L.LAUNCH_WET_RUNS, 0)
         b.step(steps)
         red = b.reduce()
     nu = (0.8 - 0.5) / 3.0
@@ -143,7 +153,8 @@ def test_autotune_keeps_the_state():
         before = [lat.populations() for lat in lats]
         seg, blocks, ms = b.autotune(steps=2)
         say(f"  autotune: R={seg}, {blocks} work-groups, {ms:.4f} ms/step")
-        assert seg in (1, 2, 4, 8) and blocks % 8 == 0 and ms > 0
+        assert seg in (1, 2, 4, 8, L.LAUNCH_WET_RUNS) and blocks % 8 == 0 and ms > 0
+        assert seg != L.LAUNCH_WET_RUNS or blocks == 0
         for x, lat in zip(before, lats):
             assert np.array_equal(x, lat.populations())
         b.step(30)
@@ -153,7 +164,8 @@ def test_autotune_keeps_the_state():
         lat.close()
 
 
-def test_time_varying_inlets_in_an_ensemble():
+@pytest.mark.parametrize("launch", ["tiles", "wet-runs"])
+def test_time_varying_inlets_in_an_ensemble(launch):
     """Channels with a velocity inlet that changes between calls, mixed with
     a porous sample: each ends where it ends alone with the same schedule."""
     wls = [W.channel(96, 256, radius=8, centre=(48, 60), re=20), W.channel(128, 192, radius=6, centre=(60, 50), re=20),
@@ -166,6 +178,8 @@ def test_time_varying_inlets_in_an_ensemble():
             lat.set_open_values(0, ux=w.inlet["ux"] * (1.0 - np.exp(-0.05 * (k + 1))), uy=w.inlet["uy"])
 
     with L.Batch(lats) as b:
+        if launch == "wet-runs":
+            b.set_launch(L.LAUNCH_WET_RUNS, 0)
         for k in range(5):
             for w, lat, lat1 in zip(wls, lats, alone):
                 set_values(lat, w, k)
@@ -196,6 +210,8 @@ def test_membership_rules():
     assert e.value.code == 4
     with pytest.raises(L.LbmError):
         b.step(1, L.STEP_MACRO)
+    with pytest.raises(L.LbmError):
+        b.set_launch(L.LAUNCH_WET_RUNS, 64)   # wet runs: one wave per run only
     b.close()
     lats[0].step(2)                     # free again
     for lat in lats:
