@@ -282,13 +282,16 @@ int lbm_get_scale(lbm_ctx *ctx, double *scale);
 /* Launch shape of the step kernel: `segments` 64-row segments per wave
  * (1, 2, 4 or 8; 0 = the storage format's default), or LBM_LAUNCH_WET_RUNS:
  * one wave per run of up to 64 consecutive wet rows of a column, so no lane
- * idles on a solid node.  Results are the same bits for every shape.  No
- * counterpart in the reference (a tuning knob of this engine). */
-enum { LBM_LAUNCH_WET_RUNS = 64 };
+ * idles on a solid node, or LBM_LAUNCH_RUN_TABLE: the same runs with every
+ * lane's neighbour addressing read from a table built on first use (4 bytes
+ * per wet node and step more traffic, far fewer instructions per wave).
+ * Results are the same bits for every shape.  No counterpart in the
+ * reference (a tuning knob of this engine). */
+enum { LBM_LAUNCH_WET_RUNS = 64, LBM_LAUNCH_RUN_TABLE = 65 };
 int lbm_set_launch(lbm_ctx *ctx, int32_t segments);
 int lbm_get_launch(lbm_ctx *ctx, int32_t *segments);
 
-/* Times each launch shape (1, 2, 4, 8 segments, wet runs) over `steps`
+/* Times each launch shape (1, 2, 4, 8 segments, wet runs, tabled wet runs) over `steps`
  * launches (every launch rewrites the spare buffer from the current state,
  * so the state does not change) and keeps the fastest.  Reports the choice
  * and its time per step (outputs may be NULL).  No counterpart in the

@@ -84,6 +84,8 @@ StepArgs make_args(const lbm_ctx *c)
     a.nruns = c->nruns;
     a.perim = c->d_perim;
     a.nperim = c->nperim;
+    a.trun = c->d_trun;
+    a.tlane = c->d_tlane;
     a.psbb = c->psbb ? 1 : 0;
     a.ps_t = c->psbb ? 1.0 / c->ps_tau : 0.0;
     a.ps_mask = c->d_ps_mask;
@@ -148,6 +150,8 @@ void free_geometry(lbm_ctx *c)
     (void)hipFree(c->d_wet); c->d_wet = nullptr;
     (void)hipFree(c->d_sb); c->d_sb = nullptr;
     (void)hipFree(c->d_runs); c->d_runs = nullptr; c->nruns = 0;
+    (void)hipFree(c->d_trun); c->d_trun = nullptr;
+    (void)hipFree(c->d_tlane); c->d_tlane = nullptr;
     (void)hipFree(c->d_perim); c->d_perim = nullptr; c->nperim = 0;
     (void)hipFree(c->d_edge_kind); c->d_edge_kind = nullptr;
     (void)hipFree(c->d_frozen); c->d_frozen = nullptr;
@@ -372,6 +376,57 @@ int stage_points(lbm_ctx *c, const int64_t *ij, const int32_t *masks, int64_t n,
     HIP_TRY(hipMemcpy(c->d_masks, hm, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice));
     c->h_pts.assign(hij, hij + 2 * n);
     c->h_masks.assign(hm, hm + n);
+    return LBM_OK;
+}
+
+// Tables of the tabled wet runs (kRunTableLaunch), built from the run table
+// on first use: per run the entry and the compact indices of its first row
+// in the neighbouring columns, per lane one record (see StepArgs::tlane).
+int ensure_run_tables(lbm_ctx *c)
+{
+    if (c->d_trun || c->nruns == 0) return LBM_OK;
+    const int64_t nr = c->nr;
+    auto word = [&](int64_t i, int64_t j) { return (size_t)((j + 1) * c->wl + 1 + i / 64); };
+    auto wet = [&](int64_t i, int64_t j) {
+        return i >= 0 && i < nr && j >= 0 && j < c->nc && ((c->h_wet[word(i, j)] >> (i % 64)) & 1);
+    };
+    // compact index of the first wet row at or below row i in column j
+    auto rank = [&](int64_t i, int64_t j) {
+        const size_t w = word(i, j);
+        const uint64_t below = (i % 64) ? (c->h_wet[w] & ((1ull << (i % 64)) - 1ull)) : 0ull;
+        return (int32_t)(c->h_sb[w] + __builtin_popcountll(below));
+    };
+    static const int di[9] = {0, 0, -1, 0, 1, -1, -1, 1, 1};
+    static const int dj[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+    std::vector<int32_t> trun((size_t)(8 * c->nruns), 0);
+    std::vector<uint32_t> tlane((size_t)(64 * c->nruns), 0u);
+    for (int64_t t = 0; t < c->nruns; ++t) {
+        const int32_t *r = &c->h_runs[(size_t)(4 * t)];
+        const int64_t j = r[0], i0 = r[1];
+        const int64_t n = r[3];
+        int32_t *e = &trun[(size_t)(8 * t)];
+        e[0] = r[0]; e[1] = r[1]; e[2] = r[2]; e[3] = r[3];
+        e[4] = rank(i0, j - 1);
+        e[5] = rank(i0, j + 1);
+        int64_t i = i0, lc = 0, rc = 0;
+        for (int64_t l = 0; l < n; ++l, ++i) {
+            while (!wet(i, j)) {  // the lane's row: the next wet row of column j
+                lc += wet(i, j - 1);
+                rc += wet(i, j + 1);
+                ++i;
+            }
+            uint32_t bits = 0;
+            for (int d = 1; d < 9; ++d)  // source of direction d: (i - di[d], j - dj[d])
+                if (wet(i - di[d], j - dj[d])) bits |= 1u << (d - 1);
+            tlane[(size_t)(64 * t + l)] = (uint32_t)(i - i0) | (uint32_t)lc << 7 | (uint32_t)rc << 14 | bits << 21;
+            lc += wet(i, j - 1);
+            rc += wet(i, j + 1);
+        }
+    }
+    HIP_TRY(hipMalloc(&c->d_trun, trun.size() * sizeof(int32_t)));
+    HIP_TRY(hipMemcpy(c->d_trun, trun.data(), trun.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&c->d_tlane, tlane.size() * sizeof(uint32_t)));
+    HIP_TRY(hipMemcpy(c->d_tlane, tlane.data(), tlane.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     return LBM_OK;
 }
 
@@ -773,6 +828,10 @@ int lbm_set_geometry(lbm_ctx *c, const uint8_t *cls)
         HIP_TRY(launch_fill_macro(c->d_macro[k], nr * nc, c->stream));
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->segments == kRunTableLaunch) {  // the chosen shape reads tables of the new geometry
+        const int r = ensure_run_tables(c);
+        if (r) return r;
+    }
     c->macro_slot = 0;
     c->has_geom = true;
     c->state = kNone;
@@ -954,8 +1013,16 @@ int lbm_step(lbm_ctx *c, int64_t n, uint32_t flags)
 int lbm_set_launch(lbm_ctx *c, int32_t segments)
 {
     if (!c) return fail(LBM_ERR_INVALID, "null context");
-    if (segments != 0 && segments != 1 && segments != 2 && segments != 4 && segments != 8 && segments != kRunLaunch)
-        return fail(LBM_ERR_INVALID, "segments must be 0, 1, 2, 4, 8 or %d (wet runs)", kRunLaunch);
+    if (segments != 0 && segments != 1 && segments != 2 && segments != 4 && segments != 8 && segments != kRunLaunch &&
+        segments != kRunTableLaunch)
+        return fail(LBM_ERR_INVALID, "segments must be 0, 1, 2, 4, 8, %d (wet runs) or %d (tabled wet runs)", kRunLaunch,
+                    kRunTableLaunch);
+    if (segments == kRunTableLaunch) {
+        if (!c->has_geom) return fail(LBM_ERR_STATE, "no geometry");
+        HIP_TRY(hipSetDevice(c->desc.device));
+        int r = ensure_run_tables(c);
+        if (r) return r;
+    }
     c->segments = segments;
     drop_graphs(c);
     return LBM_OK;
@@ -974,6 +1041,12 @@ int lbm_autotune(lbm_ctx *c, int32_t steps, int32_t *segments, double *ms_per_st
     if (steps < 1) return fail(LBM_ERR_INVALID, "steps must be >= 1");
     if (c->state == kNone) return fail(LBM_ERR_STATE, "no populations (call lbm_init_equilibrium)");
     HIP_TRY(hipSetDevice(c->desc.device));
+    {
+        // the tabled-run candidate's tables, before the arguments that point
+        // at them are formed
+        const int r = ensure_run_tables(c);
+        if (r) return r;
+    }
     hipEvent_t t0 = nullptr, t1 = nullptr;
     HIP_TRY(hipEventCreate(&t0));
     hipError_t err = hipEventCreate(&t1);
@@ -986,7 +1059,7 @@ int lbm_autotune(lbm_ctx *c, int32_t steps, int32_t *segments, double *ms_per_st
     a.fin = c->d_F[c->cur];
     a.fout = c->d_F[1 - c->cur];
     a.macro = nullptr;
-    for (int32_t R : {1, 2, 4, 8, kRunLaunch}) {
+    for (int32_t R : {1, 2, 4, 8, kRunLaunch, kRunTableLaunch}) {
         if (err == hipSuccess) err = launch_step(c->storage, true, a, c->stream, R);
         if (err == hipSuccess) err = hipEventRecord(t0, c->stream);
         for (int32_t k = 0; k < steps && err == hipSuccess; ++k) err = launch_step(c->storage, true, a, c->stream, R);

@@ -47,6 +47,8 @@ struct lbm_ctx {
     int32_t *d_runs = nullptr;  // wet-run table of the run kernel
     int64_t nruns = 0;
     std::vector<int32_t> h_runs;  // host copy (ensembles build theirs from it)
+    int32_t *d_trun = nullptr;    // tabled wet runs (built on first use)
+    uint32_t *d_tlane = nullptr;
     int32_t *d_perim = nullptr;  // perimeter items (column, segment) of the perimeter kernel
     int64_t nperim = 0;
     uint8_t *d_edge_kind = nullptr;

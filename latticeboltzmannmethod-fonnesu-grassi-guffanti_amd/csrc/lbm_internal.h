@@ -80,6 +80,16 @@ struct StepArgs {
     // column and segment index.
     const int32_t *perim;
     int64_t nperim;
+    // Tabled wet runs (kRunTableLaunch): the same runs, per run 8 int32 --
+    // column, first row, compact index of the first row, wet rows, the
+    // compact indices of the first row's left and right neighbour rows
+    // (rank in columns j-1 and j+1), 2 unused -- and per run 64 lane
+    // records (uint32): row - first row (bits 0-6), wet rows of column j-1
+    // from the first row up to the lane's row (7-13), the same for column
+    // j+1 (14-20), and the lane's source-is-wet bits of directions 1..8
+    // (21-28).
+    const int32_t *trun;
+    const uint32_t *tlane;
     // Partially saturated bounce-back of the wet obstacles (FP64 storage
     // only; every node then runs through the general path).  For a wet
     // node n and direction d whose source n - c_d is a wet obstacle, the
@@ -96,6 +106,8 @@ struct StepArgs {
 // Launch shape value selecting the run kernel: one wave per run of up to 64
 // consecutive wet rows of a column instead of R 64-row segments.
 constexpr int32_t kRunLaunch = LBM_LAUNCH_WET_RUNS;
+// The same runs with each lane's addressing read from a table.
+constexpr int32_t kRunTableLaunch = LBM_LAUNCH_RUN_TABLE;
 
 // Fills fin_plane / fout_plane from fin / fout.
 void set_planes(StepArgs &a);

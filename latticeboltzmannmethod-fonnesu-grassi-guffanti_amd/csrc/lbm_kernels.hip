@@ -1151,9 +1151,12 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
 // past the run read their first row's slot (bounce-back addresses, in
 // range) and do not store.  Loads come before any branch (see step_wave).
 template <typename T>
+__device__ __forceinline__ void run_lane(const StepArgs &a, int i, int j, int s0, int n, int lane, bool act,
+                                         int32_t rank0, int32_t rank2, uint32_t wet);
+
+template <typename T>
 __device__ __forceinline__ void run_wave(const StepArgs &a, int j, int i0, int s0, int n, int lane, int32_t *rows)
 {
-    using V = typename ComputeOf<T>::type;
     const int w0 = i0 >> 6;
     // words w0-1 .. w0+3 of columns j-1, j, j+1 and the segment bases of
     // words w0 .. w0+2.  A column keeps three zero guard words past its last
@@ -1208,20 +1211,32 @@ __device__ __forceinline__ void run_wave(const StepArgs &a, int j, int i0, int s
         const int32_t base = kk == 0 ? B[c][0] : (kk == 1 ? B[c][1] : B[c][2]);
         rank[c] = base + (int32_t)__builtin_popcountll(cur & ((1ull << bp) - 1ull));
     }
-    const int32_t own = act ? s0 + lane : s0;
-    Idx x;
-    x.src[0] = own;
-    x.src[1] = rank[0];
-    x.src[2] = own + 1;
-    x.src[3] = rank[2];
-    x.src[4] = own - 1;
-    x.src[5] = rank[0] + (int32_t)wm[0];
-    x.src[6] = rank[2] + (int32_t)wm[2];
-    x.src[7] = rank[2] - 1;
-    x.src[8] = rank[0] - 1;
     const uint32_t wet = act ? (wm[0] << 1) | (wd[1] << 2) | (wm[2] << 3) | (wu[1] << 4) | (wd[0] << 5) |
                                    (wd[2] << 6) | (wu[2] << 7) | (wu[0] << 8)
                              : 0u;
+    run_lane<T>(a, i, j, s0, n, lane, act, rank[0], rank[2], wet);
+}
+
+// The rest of a wet-run wave once a lane knows its row i, whether it holds a
+// node (act), the compact indices of its row in columns j-1 and j+1 (rank0,
+// rank2: the first wet row at or below row i there) and its source-is-wet
+// bits (bit d for direction d): gather, collide, store.
+template <typename T>
+__device__ __forceinline__ void run_lane(const StepArgs &a, int i, int j, int s0, int n, int lane, bool act,
+                                         int32_t rank0, int32_t rank2, uint32_t wet)
+{
+    using V = typename ComputeOf<T>::type;
+    const int32_t own = act ? s0 + lane : s0;
+    Idx x;
+    x.src[0] = own;
+    x.src[1] = rank0;
+    x.src[2] = own + 1;
+    x.src[3] = rank2;
+    x.src[4] = own - 1;
+    x.src[5] = rank0 + (int32_t)((wet >> 1) & 1u);
+    x.src[6] = rank2 + (int32_t)((wet >> 3) & 1u);
+    x.src[7] = rank2 - 1;
+    x.src[8] = rank0 - 1;
     T raw[9];
     gather_interior_by(a, x, [wet](int d) { return ((wet >> d) & 1u) != 0; }, raw);
     // All nine loads issue before any of their values is used: without
@@ -1281,6 +1296,35 @@ __global__ __launch_bounds__(kBlockRows) void run_kernel(StepArgs a)
     run_wave<T>(a, __builtin_amdgcn_readfirstlane(tp[0]), __builtin_amdgcn_readfirstlane(tp[1]),
                 __builtin_amdgcn_readfirstlane(tp[2]), __builtin_amdgcn_readfirstlane(tp[3]), threadIdx.x & 63,
                 rows[wave]);
+}
+
+// The wet runs with every lane's addressing read from the tables built with
+// the geometry (StepArgs::trun, tlane): the run entry (scalar) and the
+// lane's record (one 4-byte vector load) are the only loads before the
+// population loads, and they issue together.
+template <typename T>
+__global__ __launch_bounds__(kBlockRows) void tab_kernel(StepArgs a)
+{
+    asm volatile("" ::"s"(a.nblocks), "s"(a.nruns), "s"(a.trun), "s"(a.tlane));  // one round trip
+    const int64_t b = xcd_remap(blockIdx.x, (uint32_t)a.nblocks);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t t = b * (kBlockRows / 64) + wave;
+    if (t >= a.nruns) return;
+    const int lane = threadIdx.x & 63;
+    const LBM_CONST int32_t *tp = (const LBM_CONST int32_t *)a.trun + 8 * t;
+    const uint32_t rec = ((const LBM_GLOBAL uint32_t *)a.tlane)[64 * t + lane];
+    const int j = __builtin_amdgcn_readfirstlane(tp[0]);
+    const int i0 = __builtin_amdgcn_readfirstlane(tp[1]);
+    const int s0 = __builtin_amdgcn_readfirstlane(tp[2]);
+    const int n = __builtin_amdgcn_readfirstlane(tp[3]);
+    const int32_t bl = __builtin_amdgcn_readfirstlane(tp[4]);
+    const int32_t br = __builtin_amdgcn_readfirstlane(tp[5]);
+    const bool act = lane < n;
+    const int i = i0 + (int)(rec & 127u);
+    const int32_t rank0 = bl + (int32_t)((rec >> 7) & 127u);
+    const int32_t rank2 = br + (int32_t)((rec >> 14) & 127u);
+    const uint32_t wet = act ? ((rec >> 21) & 255u) << 1 : 0u;
+    run_lane<T>(a, i, j, s0, n, lane, act, rank0, rank2, wet);
 }
 
 // The perimeter share of a gathering step of one lattice: one wave per
@@ -1741,6 +1785,16 @@ hipError_t launch_step_t(bool gather, StepArgs a, hipStream_t s)
 }
 
 template <typename T>
+hipError_t launch_step_tab(StepArgs a, hipStream_t s)
+{
+    hipError_t e = hipSuccess;
+    if (a.nruns > 0 && (!a.trun || !a.tlane)) return hipErrorInvalidValue;  // tables not built
+    if (a.nruns > 0) e = launch(tab_kernel<T>, (a.nruns + kBlockRows / 64 - 1) / (kBlockRows / 64), s, a);
+    if (e != hipSuccess) return e;
+    return launch_perimeter<T>(a, s);
+}
+
+template <typename T>
 hipError_t launch_step_runs(StepArgs a, hipStream_t s)
 {
     hipError_t e = hipSuccess;
@@ -1752,8 +1806,8 @@ hipError_t launch_step_runs(StepArgs a, hipStream_t s)
 template <typename T>
 hipError_t launch_step_r(bool gather, int32_t R, const StepArgs &a, hipStream_t s)
 {
-    if (R == kRunLaunch) {
-        if (gather) return launch_step_runs<T>(a, s);
+    if (R == kRunLaunch || R == kRunTableLaunch) {
+        if (gather) return R == kRunLaunch ? launch_step_runs<T>(a, s) : launch_step_tab<T>(a, s);
         R = default_segments(sizeof(T) == 8 ? kF64 : kF32);
     }
     switch (R) {

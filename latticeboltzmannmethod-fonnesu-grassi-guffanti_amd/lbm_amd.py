@@ -29,8 +29,10 @@ ZOUHE_VELOCITY, ZOUHE_PRESSURE, OPEN_NOOP = 0, 1, 2
 F64, F32_SHIFTED, F16_SHIFTED = 0, 1, 2
 STEP_MACRO, STEP_CHECK, STEP_NO_GRAPH = 1, 2, 4
 OPT_F16_ROUND_TO_NEAREST = 1
-# lbm_set_launch shape: one wave per run of up to 64 consecutive wet rows
+# lbm_set_launch shapes: one wave per run of up to 64 consecutive wet rows;
+# the same runs with each lane's addressing read from a table
 LAUNCH_WET_RUNS = 64
+LAUNCH_RUN_TABLE = 65
 ERR_NAMES = {0: "OK", 1: "INVALID", 2: "HIP", 3: "NAN", 4: "STATE", 5: "UNSUPPORTED", 6: "NOMEM", 7: "RANGE"}
 ERR_INVALID, ERR_STATE, ERR_NAN, ERR_UNSUPPORTED, ERR_RANGE = 1, 4, 3, 5, 7
 

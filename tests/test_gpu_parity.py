@@ -223,13 +223,13 @@ def test_autotune_keeps_state_and_results():
     for storage in (L.F64, L.F32_SHIFTED, L.F16_SHIFTED):
         ref = wl.make_lattice(storage)
         ref.step(80)
-        for seg in (1, 2, 4, 8, L.LAUNCH_WET_RUNS):
+        for seg in (1, 2, 4, 8, L.LAUNCH_WET_RUNS, L.LAUNCH_RUN_TABLE):
             lat = wl.make_lattice(storage)
             lat.step(30)
             before = lat.populations()
             if seg == 1:
                 chosen, ms = lat.autotune(steps=2)
-                assert chosen in (1, 2, 4, 8, L.LAUNCH_WET_RUNS) and ms > 0
+                assert chosen in (1, 2, 4, 8, L.LAUNCH_WET_RUNS, L.LAUNCH_RUN_TABLE) and ms > 0
             assert np.array_equal(before, lat.populations())
             lat.set_launch(seg)
             lat.step(50)
