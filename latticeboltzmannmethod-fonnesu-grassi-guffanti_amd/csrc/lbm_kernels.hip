@@ -298,11 +298,36 @@ __device__ __forceinline__ void gather_interior_by(const StepArgs &a, const Idx 
     }
 }
 
-// Per-lane mask form (bit d of m = the source of direction d is wet).
+// (s & x) | (~s & y) in one instruction.  Written out because the compiler
+// turns the plain expression back into a test and a select when s is a
+// sign-extended bit.
+__device__ __forceinline__ uint32_t bit_insert(uint32_t s, uint32_t x, uint32_t y)
+{
+    uint32_t r;
+    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "v"(s), "v"(x), "v"(y));
+    return r;
+}
+
+// Per-lane mask form (bit d of m = the source of direction d is wet): bit
+// d, sign-extended, selects between the two offsets by a bit-field insert
+// (two instructions per direction where a test and a select take three).
 template <typename T>
 __device__ __forceinline__ void gather_interior(const StepArgs &a, const Idx &x, uint32_t m, T *p)
 {
-    gather_interior_by(a, x, [m](int d) { return ((m >> d) & 1) != 0; }, p);
+    const uint32_t E = (uint32_t)sizeof(T);
+    const uint32_t twoP = 2u * (uint32_t)a.g.plane * E;
+    const uint32_t own = (uint32_t)x.src[0] * E;
+    p[0] = ld((const T *)a.fin_plane[0], own);
+#pragma unroll
+    for (int d = 1; d < 9; ++d) {
+        const uint32_t sel = (uint32_t)((int32_t)(m << (31 - d)) >> 31);  // all ones: the source is wet
+        const int o = kOpp[d];
+        const bool low = d < o;
+        const T *base = (const T *)a.fin_plane[low ? d : o];
+        const uint32_t nb = (uint32_t)x.src[d] * E + (low ? 0u : twoP);
+        const uint32_t ob = own + (low ? twoP : 0u);
+        p[d] = ld(base, bit_insert(sel, nb, ob));
+    }
 }
 
 // Wave-uniform source masks (bit l of src[d] = the source of direction d of
@@ -1220,12 +1245,12 @@ __device__ __forceinline__ void run_wave(const StepArgs &a, int j, int i0, int s
 // The rest of a wet-run wave once a lane knows its row i, whether it holds a
 // node (act), the compact indices of its row in columns j-1 and j+1 (rank0,
 // rank2: the first wet row at or below row i there) and its source-is-wet
-// bits (bit d for direction d): gather, collide, store.
+// bits (bit d for direction d): gather (run_gather), then collide and store
+// (run_finish).
 template <typename T>
-__device__ __forceinline__ void run_lane(const StepArgs &a, int i, int j, int s0, int n, int lane, bool act,
-                                         int32_t rank0, int32_t rank2, uint32_t wet)
+__device__ __forceinline__ void run_gather(const StepArgs &a, int s0, int lane, bool act, int32_t rank0, int32_t rank2,
+                                           uint32_t wet, T *raw)
 {
-    using V = typename ComputeOf<T>::type;
     const int32_t own = act ? s0 + lane : s0;
     Idx x;
     x.src[0] = own;
@@ -1237,15 +1262,16 @@ __device__ __forceinline__ void run_lane(const StepArgs &a, int i, int j, int s0
     x.src[6] = rank2 + (int32_t)((wet >> 3) & 1u);
     x.src[7] = rank2 - 1;
     x.src[8] = rank0 - 1;
-    T raw[9];
-    gather_interior_by(a, x, [wet](int d) { return ((wet >> d) & 1u) != 0; }, raw);
-    // All nine loads issue before any of their values is used: without
-    // this the scheduler interleaves the first moments with the later
-    // loads and waits for the early data in between, so a wave spends
-    // several memory round trips where one suffices.
-    asm volatile("" : "+v"(raw[0]), "+v"(raw[1]), "+v"(raw[2]), "+v"(raw[3]), "+v"(raw[4]), "+v"(raw[5]),
-                 "+v"(raw[6]), "+v"(raw[7]), "+v"(raw[8]));
+    gather_interior(a, x, wet, raw);
+}
+
+template <typename T>
+__device__ __forceinline__ void run_finish(const StepArgs &a, int i, int j, int s0, int n, int lane, bool act,
+                                           const T *raw)
+{
+    using V = typename ComputeOf<T>::type;
     if (!act) return;
+    const int32_t own = s0 + lane;
     V p[9];
 #pragma unroll
     for (int d = 0; d < 9; ++d) p[d] = (V)raw[d];  // FP16: scaled, see collide_shifted
@@ -1260,6 +1286,21 @@ __device__ __forceinline__ void run_lane(const StepArgs &a, int i, int j, int s0
         collide_shifted<sizeof(T) == 2>(a, p, -1, 0, i, j, out);
         encode_store<T>(a, own, out, lines);
     }
+}
+
+template <typename T>
+__device__ __forceinline__ void run_lane(const StepArgs &a, int i, int j, int s0, int n, int lane, bool act,
+                                         int32_t rank0, int32_t rank2, uint32_t wet)
+{
+    T raw[9];
+    run_gather<T>(a, s0, lane, act, rank0, rank2, wet, raw);
+    // All nine loads issue before any of their values is used: without
+    // this the scheduler interleaves the first moments with the later
+    // loads and waits for the early data in between, so a wave spends
+    // several memory round trips where one suffices.
+    asm volatile("" : "+v"(raw[0]), "+v"(raw[1]), "+v"(raw[2]), "+v"(raw[3]), "+v"(raw[4]), "+v"(raw[5]),
+                 "+v"(raw[6]), "+v"(raw[7]), "+v"(raw[8]));
+    run_finish<T>(a, i, j, s0, n, lane, act, raw);
 }
 
 // The time-step kernel of one lattice: wave w of the (XCD-remapped) grid
@@ -1301,7 +1342,10 @@ __global__ __launch_bounds__(kBlockRows) void run_kernel(StepArgs a)
 // The wet runs with every lane's addressing read from the tables built with
 // the geometry (StepArgs::trun, tlane): the run entry (scalar) and the
 // lane's record (one 4-byte vector load) are the only loads before the
-// population loads, and they issue together.
+// population loads, and they issue together.  (Two runs per wave, the loads
+// of both in flight together, measured no faster for FP16 and slower for
+// FP32 on the 8192^2 porous medium: the wave count per second is not
+// limited by the bytes in flight.)
 template <typename T>
 __global__ __launch_bounds__(kBlockRows) void tab_kernel(StepArgs a)
 {

@@ -51,7 +51,8 @@ def run_case(name, cls, collision, inlet_policy, outlet_policy, steps, seed=0, t
         f"inlet {inlet_policy}, outlet {outlet_policy}")
     # the same run with row-segment waves (default shape), wet-run waves and
     # tabled wet-run waves
-    gpus = [wl.make_lattice(L.F64), wl.make_lattice(L.F64), wl.make_lattice(L.F64)]
+    shapes = ("row segments", "wet runs", "tabled wet runs")
+    gpus = [wl.make_lattice(L.F64) for _ in shapes]
     gpus[1].set_launch(L.LAUNCH_WET_RUNS)
     gpus[2].set_launch(L.LAUNCH_RUN_TABLE)
     orc = oracle_from(wl, threads=THREADS)
@@ -71,7 +72,7 @@ def run_case(name, cls, collision, inlet_policy, outlet_policy, steps, seed=0, t
     orc.step(steps - half)
     po = orc.populations()
     assert np.all(np.isfinite(po[:, wet_mask(cls)])), f"{name}: the oracle run itself is not finite"
-    for shape, gpu in zip(("row segments", "wet runs", "tabled wet runs"), gpus):
+    for shape, gpu in zip(shapes, gpus):
         err = max_abs_wet(gpu.populations(), po, cls)
         say(f"  {shape}: max|dP| = {err:.3e}")
         assert err <= TOL, f"{name} ({shape}): {err}"
