@@ -100,6 +100,18 @@ def copy_ceiling(nbytes, reps=20):
                 gbs=L.copy_bandwidth(nbytes, reps))
 
 
+def access_patterns(nr, nc, reps=20):
+    """The step's possible access patterns without arithmetic, on a dense
+    nr x nc periodic lattice, per element size (lbm_pattern_bandwidth): the
+    two-buffer pull the kernels use, the AA in-place pattern (even/odd
+    alternating, and each step alone), and the pull in a blocked layout."""
+    out = {}
+    for name, size in (("f16", 2), ("f32", 4), ("f64", 8)):
+        out[name] = {L.PATTERN_NAMES[p]: round(L.pattern_bandwidth(p, size, nr, nc, reps), 1)
+                     for p in sorted(L.PATTERN_NAMES)}
+    return out
+
+
 def build_workload(name):
     if name == "porous8192":
         return W.porous(8192)
@@ -372,11 +384,21 @@ def main():
     ap.add_argument("--no-c4", action="store_true", help="skip the 4096^2 TRT cavity line")
     ap.add_argument("--c4-error-steps", type=int, default=200000,
                     help="steps after which the 4096^2 TRT cavity's FP32/FP16 errors are stated (0 = skip)")
+    ap.add_argument("--no-patterns", action="store_true", help="skip the access-pattern ceilings")
+    ap.add_argument("--patterns-only", action="store_true",
+                    help="print the access-pattern ceilings on the workload's NR x NC and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU slab decomposition is a design, not built)", file=sys.stderr)
         sys.exit(2)
+
+    if args.patterns_only:
+        wl = build_workload(args.workload)
+        nr, nc = wl.nr, wl.nc
+        print(json.dumps({"access_patterns": access_patterns(nr, nc), "nr": nr, "nc": nc,
+                          "unit": "GB/s (18 elements moved per node)"}), flush=True)
+        return
 
     ev = HipEvents()
     t0 = time.perf_counter()
@@ -387,11 +409,18 @@ def main():
     # the copy ceiling over the FP32 step's volume (one population buffer)
     ceiling = copy_ceiling(W.algorithmic_bytes(wl, L.F32_SHIFTED) // 2)
     log(f"  device copy ceiling: {ceiling['gbs']:.0f} GB/s ({ceiling['bytes'] / 1e9:.2f} GB copies)")
+    patterns = None
+    if not args.no_patterns and (wl.nr * wl.nc) % 64 == 0:
+        patterns = access_patterns(wl.nr, wl.nc)
+        for s, p in patterns.items():
+            log(f"  access patterns {s}: " + ", ".join(f"{k} {v:.0f}" for k, v in p.items()) + " GB/s")
     order = [head] if args.only else [head] + [s for s in VARIANTS if s != head]
     results, lats = {}, {}
     for s in order:
         lat, r = time_storage(wl, s, args.steps, args.warmup, ev)
         r["frac_of_copy"] = r["achieved_gbs"] / ceiling["gbs"]
+        if patterns:
+            r["frac_of_pattern"] = r["achieved_gbs"] / patterns[s[:3]]["ab_pull"]
         results[s] = r
         log(f"  {r['storage']}: {r['mlups']:.1f} MLUPS, {r['ms_per_step']:.4f} ms/step, "
             f"{r['achieved_gbs']:.0f} GB/s = {100 * r['frac']:.1f}% of 8 TB/s, "
@@ -438,6 +467,7 @@ def main():
                      "frac": round(r["frac"], 4), "traffic": r["traffic_bytes_per_step"]},
         "cpu_baseline": cpu,
         "copy_ceiling": {k: (round(v, 1) if isinstance(v, float) else v) for k, v in ceiling.items()},
+        "access_patterns": patterns,
         "storages": {s: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in x.items()}
                      for s, x in results.items()},
         "error_vs_f64": errors,

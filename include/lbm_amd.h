@@ -370,6 +370,25 @@ int lbm_batch_synchronize(lbm_batch *batch);
  * bandwidth against it. */
 int lbm_copy_bandwidth(int32_t device, int64_t bytes, int32_t reps, double *gbs);
 
+/* The rate (read + write bytes per second / 1e9, 18 elements moved per
+ * node) of a D2Q9 step's access pattern without its arithmetic, on a dense
+ * n_rows x n_cols lattice with periodic wrap (n_rows * n_cols a multiple of
+ * 64), `elem_bytes` 2, 4 or 8, timed over `reps` launches:
+ *   LBM_PATTERN_AB       pull between two buffers of 9 planes (the step
+ *                        kernels' layout);
+ *   LBM_PATTERN_AA       the AA in-place pattern on one buffer, even and
+ *                        odd steps alternating;
+ *   LBM_PATTERN_AA_EVEN  its even step alone (own slots, swapped);
+ *   LBM_PATTERN_AA_ODD   its odd step alone (neighbours' slots);
+ *   LBM_PATTERN_BLOCKED  pull with the 9 populations of 64 consecutive
+ *                        nodes stored together.
+ * Allocates and frees two buffers of 9 * n_rows * n_cols * elem_bytes
+ * bytes.  No counterpart in the reference (a measurement of this engine's
+ * layout choices). */
+enum { LBM_PATTERN_AB = 0, LBM_PATTERN_AA = 1, LBM_PATTERN_AA_EVEN = 2, LBM_PATTERN_AA_ODD = 3, LBM_PATTERN_BLOCKED = 4 };
+int lbm_pattern_bandwidth(int32_t device, int32_t pattern, int32_t elem_bytes, int64_t n_rows, int64_t n_cols,
+                          int32_t reps, double *gbs);
+
 /* ------------------------------------------------------------------ */
 /* Geometry helpers (host only; write a class plane, ColMajor NR*NC)   */
 /* ------------------------------------------------------------------ */

@@ -47,7 +47,7 @@ EXPORTED = [
     "lbm_get_scale", "lbm_geometry_cavity", "lbm_geometry_channel", "lbm_geometry_add_disc",
     "lbm_geometry_porous", "lbm_set_launch", "lbm_get_launch", "lbm_autotune", "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_set_launch", "lbm_batch_get_launch",
     "lbm_batch_autotune", "lbm_batch_step", "lbm_batch_reduce", "lbm_batch_info", "lbm_batch_get_stream",
-    "lbm_batch_synchronize", "lbm_copy_bandwidth",
+    "lbm_batch_synchronize", "lbm_copy_bandwidth", "lbm_pattern_bandwidth",
 ]
 
 
@@ -132,6 +132,7 @@ def lib():
             "lbm_batch_get_stream": [vp, C.POINTER(vp)],
             "lbm_batch_synchronize": [vp],
             "lbm_copy_bandwidth": [C.c_int32, C.c_int64, C.c_int32, dp],
+            "lbm_pattern_bandwidth": [C.c_int32, C.c_int32, C.c_int32, C.c_int64, C.c_int64, C.c_int32, dp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -192,6 +193,21 @@ def copy_bandwidth(nbytes, reps=20, device=0):
     GB/s) of the library's copy kernel over nbytes (lbm_copy_bandwidth)."""
     gbs = C.c_double(0)
     _check(lib().lbm_copy_bandwidth(device, int(nbytes), reps, C.byref(gbs)))
+    return gbs.value
+
+
+# access patterns of lbm_pattern_bandwidth
+PATTERN_AB, PATTERN_AA, PATTERN_AA_EVEN, PATTERN_AA_ODD, PATTERN_BLOCKED = 0, 1, 2, 3, 4
+PATTERN_NAMES = {PATTERN_AB: "ab_pull", PATTERN_AA: "aa", PATTERN_AA_EVEN: "aa_even", PATTERN_AA_ODD: "aa_odd",
+                 PATTERN_BLOCKED: "blocked_pull"}
+
+
+def pattern_bandwidth(pattern, elem_bytes, nr, nc, reps=20, device=0):
+    """Rate (read + write GB/s, 18 elements per node) of a D2Q9 step's access
+    pattern without arithmetic on a dense nr x nc periodic lattice
+    (lbm_pattern_bandwidth)."""
+    gbs = C.c_double(0)
+    _check(lib().lbm_pattern_bandwidth(device, pattern, elem_bytes, int(nr), int(nc), reps, C.byref(gbs)))
     return gbs.value
 
 

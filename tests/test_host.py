@@ -115,6 +115,16 @@ def test_copy_probe_rejects_bad_sizes_before_touching_a_device():
         assert e.value.code == 1
 
 
+def test_pattern_probe_rejects_bad_arguments_before_touching_a_device():
+    """lbm_pattern_bandwidth validates its arguments first (no HIP call)."""
+    bad = ((9, 4, 64, 64, 5), (L.PATTERN_AB, 3, 64, 64, 5), (L.PATTERN_AA, 4, 1, 64, 5),
+           (L.PATTERN_AA, 4, 63, 3, 5), (L.PATTERN_BLOCKED, 8, 64, 64, 1), (L.PATTERN_AB, 8, 1 << 20, 1 << 20, 5))
+    for pattern, size, nr, nc, reps in bad:
+        with pytest.raises(L.LbmError) as e:
+            L.pattern_bandwidth(pattern, size, nr, nc, reps)
+        assert e.value.code == 1
+
+
 def test_bench_refuses_more_than_one_gpu():
     """bench.py runs on one GPU: --gpus N > 1 exits with a message before
     any device is touched."""
