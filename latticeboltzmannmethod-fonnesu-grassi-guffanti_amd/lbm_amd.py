@@ -254,6 +254,15 @@ class Lattice:
     def set_relaxation(self, tau=1.0, tau_even=1.0, tau_odd=1.0):
         _check(lib().lbm_set_relaxation(self._h, tau, tau_even, tau_odd))
 
+    def set_geometry(self, node_class):
+        """Installs a new class plane of the same dimensions (lbm_set_geometry):
+        the populations are dropped (call init_equilibrium or
+        upload_populations next); the launch shape is kept."""
+        cls = np.ascontiguousarray(node_class, dtype=np.uint8)
+        if cls.shape != (self.nc, self.nr):
+            raise ValueError(f"class plane must be {(self.nc, self.nr)} (NC, NR), got {cls.shape}")
+        _check(lib().lbm_set_geometry(self._h, cls.ctypes.data))
+
     def set_partial_bounce_back(self, tau, b=None):
         """Partially saturated bounce-back on the wet obstacles (FP64):
         the policy's tau and one weight per wet obstacle in list order

@@ -238,6 +238,48 @@ def test_autotune_keeps_state_and_results():
         ref.close()
 
 
+def test_new_geometry_under_the_tabled_shape():
+    """lbm_set_geometry on a context whose launch shape reads run tables
+    (LBM_LAUNCH_RUN_TABLE) rebuilds the tables for the new geometry: the run
+    on the second geometry is bit for bit the run of a fresh lattice with
+    that shape, and matches the oracle."""
+    a = W.porous(256, seed=5, nc=320)
+    b = W.porous(256, seed=9, nc=320)
+    assert not np.array_equal(a.node_class, b.node_class)
+    say("case new geometry on a tabled-run context, porous 256x320 (two obstacle sets), 200 steps")
+    lat = a.make_lattice(L.F64)
+    lat.set_launch(L.LAUNCH_RUN_TABLE)
+    lat.step(20)
+    lat.set_geometry(b.node_class)
+    assert lat.launch() == L.LAUNCH_RUN_TABLE
+    lat.set_open_values(0, **b.inlet)
+    lat.set_open_values(1, **b.outlet)
+    lat.init_equilibrium()
+    lat.step(200)
+    fresh = b.make_lattice(L.F64)
+    fresh.set_launch(L.LAUNCH_RUN_TABLE)
+    fresh.step(200)
+    assert np.array_equal(lat.populations(), fresh.populations())
+    orc = oracle_from(b, threads=THREADS)
+    orc.step(200)
+    compare_state(lat, orc, b.node_class, "second geometry, tabled runs")
+    for x in (lat, fresh, orc):
+        x.close()
+
+
+def test_pattern_probe_measures_plausible_rates():
+    """The access-pattern ceilings the benchmark reports: every pattern and
+    element size gives a finite rate between a floor no working HBM stream
+    misses and the part's 8 TB/s nominal peak (4096^2 lattice: two buffers
+    of 0.6-2.4 GB together, beyond the last-level cache)."""
+    say("case access-pattern probe, 4096x4096, every pattern and element size")
+    for size in (2, 4, 8):
+        for p in sorted(L.PATTERN_NAMES):
+            gbs = L.pattern_bandwidth(p, size, 4096, 4096, reps=10)
+            say(f"  {L.PATTERN_NAMES[p]} {size} B: {gbs:.0f} GB/s")
+            assert np.isfinite(gbs) and 200.0 < gbs < 8000.0, (p, size, gbs)
+
+
 def test_copy_probe_measures_a_plausible_rate():
     """The copy-kernel ceiling the benchmark reports: positive, below the
     part's 8 TB/s nominal peak, and above a floor no working HBM stream
