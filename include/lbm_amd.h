@@ -291,11 +291,12 @@ enum { LBM_LAUNCH_WET_RUNS = 64, LBM_LAUNCH_RUN_TABLE = 65 };
 int lbm_set_launch(lbm_ctx *ctx, int32_t segments);
 int lbm_get_launch(lbm_ctx *ctx, int32_t *segments);
 
-/* Times each launch shape (1, 2, 4, 8 segments, wet runs, tabled wet runs) over `steps`
- * launches (every launch rewrites the spare buffer from the current state,
- * so the state does not change) and keeps the fastest.  Reports the choice
- * and its time per step (outputs may be NULL).  No counterpart in the
- * reference. */
+/* Times each launch shape (1, 2, 4, 8 segments, wet runs, tabled wet runs)
+ * over `steps` launches (every launch rewrites the spare buffer from the
+ * current state, so the state does not change) and keeps the fastest.
+ * Reports the choice and its time per step (outputs may be NULL).
+ * LBM_ERR_STATE for a context that belongs to an ensemble (the ensemble
+ * has its own, lbm_batch_autotune).  No counterpart in the reference. */
 int lbm_autotune(lbm_ctx *ctx, int32_t steps, int32_t *segments, double *ms_per_step);
 
 /* ------------------------------------------------------------------ */

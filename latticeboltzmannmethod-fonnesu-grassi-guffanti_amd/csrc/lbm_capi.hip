@@ -423,10 +423,21 @@ int ensure_run_tables(lbm_ctx *c)
             rc += wet(i, j + 1);
         }
     }
-    HIP_TRY(hipMalloc(&c->d_trun, trun.size() * sizeof(int32_t)));
-    HIP_TRY(hipMemcpy(c->d_trun, trun.data(), trun.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIP_TRY(hipMalloc(&c->d_tlane, tlane.size() * sizeof(uint32_t)));
-    HIP_TRY(hipMemcpy(c->d_tlane, tlane.data(), tlane.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    // both tables or neither: the context takes them only once both are on
+    // the device
+    int32_t *drun = nullptr;
+    uint32_t *dlane = nullptr;
+    hipError_t e = hipMalloc(&drun, trun.size() * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMalloc(&dlane, tlane.size() * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemcpy(drun, trun.data(), trun.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dlane, tlane.data(), tlane.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(drun);
+        (void)hipFree(dlane);
+        return fail(e == hipErrorOutOfMemory ? LBM_ERR_NOMEM : LBM_ERR_HIP, "run tables: %s", hipGetErrorString(e));
+    }
+    c->d_trun = drun;
+    c->d_tlane = dlane;
     return LBM_OK;
 }
 
@@ -1039,6 +1050,7 @@ int lbm_autotune(lbm_ctx *c, int32_t steps, int32_t *segments, double *ms_per_st
 {
     if (!c) return fail(LBM_ERR_INVALID, "null context");
     if (steps < 1) return fail(LBM_ERR_INVALID, "steps must be >= 1");
+    if (c->batch) return fail(LBM_ERR_STATE, "context belongs to an ensemble: tune it with lbm_batch_autotune");
     if (c->state == kNone) return fail(LBM_ERR_STATE, "no populations (call lbm_init_equilibrium)");
     HIP_TRY(hipSetDevice(c->desc.device));
     {

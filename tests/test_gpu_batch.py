@@ -199,6 +199,9 @@ def test_membership_rules():
     with pytest.raises(L.LbmError) as e:
         lats[0].step(1)
     assert e.value.code == 4
+    with pytest.raises(L.LbmError) as e:
+        lats[0].autotune(steps=1)       # the ensemble tunes itself (lbm_batch_autotune)
+    assert e.value.code == 4
     with pytest.raises(L.LbmError):
         lats[0].close()
     with pytest.raises(L.LbmError):
