@@ -679,25 +679,39 @@ int lbm_set_geometry(lbm_ctx *c, const uint8_t *cls)
             if (k == LBM_INLET || k == LBM_OUTLET) edge[(size_t)edge_slot_host(i, j, nr, nc)] = k == LBM_INLET ? kEdgeInlet : kEdgeOutlet;
         }
     // Compact storage: wet nodes only, column-major.  sb holds the compact
-    // index of the first wet node of every (column, 64-row word).  Each
-    // column starts at the first index congruent to the row of its first
-    // wet node modulo 64 (at most 63 unused slots per column), so a run of
-    // wet rows from there on fills whole, aligned 64-element groups: wave
-    // loads and stores cover full cache lines in the unobstructed parts of
-    // the domain.  Nodes are contiguous inside a column, which is all the
-    // neighbour arithmetic (row i +- 1 = compact index +- 1) relies on.
+    // index of the first wet node of every (column, 64-row word).  A column
+    // whose wet rows form one unbroken stretch of at least kAlignRows rows
+    // starts at the first index congruent to the row of its first wet node
+    // modulo 64 (at most 63 unused slots), so its 64-row segments fill
+    // whole, aligned 64-element groups: wave loads and stores cover full
+    // cache lines and stream.  Every other column follows the previous one
+    // directly: behind the first obstacle its slots are misaligned anyway,
+    // and packing measured faster there (a 512-lattice porous ensemble
+    // +18 %, lattices of 128^2 +46 %, a porous 8192^2 +1 %), while aligning
+    // the unbroken columns of an open 8192^2 channel is worth 4 %.  Nodes
+    // are contiguous inside a column, which is all the neighbour arithmetic
+    // (row i +- 1 = compact index +- 1) relies on.
     std::vector<int32_t> sb((size_t)((nc + 2) * c->wl), 0);
     int64_t slots = 0;
     {
+        constexpr int64_t kAlignRows = 256;
         int64_t acc = 0;
         for (int64_t j = 0; j < nc; ++j) {
             const size_t w0 = (size_t)((j + 1) * c->wl + 1);
-            for (int64_t k = 0; k < nrw; ++k)
-                if (wet[w0 + k]) {
-                    const int64_t r0 = 64 * k + __builtin_ctzll(wet[w0 + k]);
-                    acc += ((r0 - acc) % 64 + 64) % 64;
-                    break;
-                }
+            // first wet row, number of wet rows and number of unbroken
+            // stretches of the column
+            int64_t r0 = -1, count = 0, stretches = 0;
+            bool prev = false;  // row 64 k - 1 wet
+            for (int64_t k = 0; k < nrw; ++k) {
+                const uint64_t m = wet[w0 + k];
+                if (m && r0 < 0) r0 = 64 * k + __builtin_ctzll(m);
+                count += __builtin_popcountll(m);
+                // stretch starts: wet rows whose row above is dry
+                const uint64_t above = (m << 1) | (prev ? 1ull : 0ull);
+                stretches += __builtin_popcountll(m & ~above);
+                prev = (m >> 63) & 1;
+            }
+            if (stretches == 1 && count >= kAlignRows) acc += ((r0 - acc) % 64 + 64) % 64;
             for (int64_t k = 0; k < nrw; ++k) {
                 sb[w0 + k] = (int32_t)acc;
                 acc += __builtin_popcountll(wet[w0 + k]);
