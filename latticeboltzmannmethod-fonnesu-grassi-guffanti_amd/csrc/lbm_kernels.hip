@@ -80,15 +80,29 @@ enum : int { BT_NONE = 0, BT_TOP, BT_RIGHT, BT_BOTTOM, BT_LEFT, BT_CORNER };
 #define LBM_GLOBAL __attribute__((address_space(1)))
 #define LBM_CONST __attribute__((address_space(4)))
 
-// Spreads the blocks of a launch so that each XCD (blocks are dealt
-// round-robin to the 8 XCDs) gets one contiguous run of the column-major
-// tile order: neighbouring row segments of a column then share an L2.
+// Maps a launch's blocks (dealt round-robin to the 8 XCDs by the hardware)
+// onto the column-major tile order in chunks: XCD x takes chunks x, x + 8,
+// x + 16, ... of `chunk` consecutive blocks, so each XCD works through a few
+// neighbouring columns at a time (their row segments share its L2) and all
+// eight move through memory together.  Blocks past the last whole group of
+// 8 chunks keep their order.  The row-segment step kernels use chunks of 8
+// columns, the others (wet runs, tables, ensembles, set-up) 256 blocks.
+// Measured against one contiguous eighth of the tile order per XCD, one
+// process per variant, best launch shape per case: porous 8192^2 FP32 +5 %,
+// FP64 +5 %, FP16 +3 %; open 8192^2 FP32 +2 %, FP16 +5 %; 4096^2 cavity
+// FP32 +5 %, FP16 +6 %, FP64 +5 %; 512-lattice ensemble +3 %.  Chunks of
+// 256 blocks for the row segments too lost 3-4 % on the cavity's FP32 and
+// FP64 shapes with few blocks per column; 8 columns for the wet runs lost 1 %.
 // Block counts stay below 2^31 (the geometry caps the 64-row segments at
 // 2^30), so the arithmetic is 32-bit.
-__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb)
+constexpr uint32_t kXcdChunk = 256;
+
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb, uint32_t chunk = kXcdChunk)
 {
-    const uint32_t q = nb >> 3, r = nb & 7, x = b & 7;
-    return x * q + (x < r ? x : r) + (b >> 3);
+    const uint32_t whole = (nb / (8 * chunk)) * (8 * chunk);
+    if (b >= whole) return b;
+    const uint32_t x = b & 7, k = b >> 3;
+    return ((k / chunk) * 8 + x) * chunk + (k % chunk);
 }
 
 __device__ __forceinline__ int64_t edge_slot(int i, int j, int nr, int nc)
@@ -1311,7 +1325,7 @@ __device__ __forceinline__ void run_lane(const StepArgs &a, int i, int j, int s0
 template <typename T, bool GATHER, int R>
 __global__ __launch_bounds__(kBlockRows) void step_kernel(StepArgs a)
 {
-    const int64_t b = xcd_remap(blockIdx.x, (uint32_t)a.nblocks);
+    const int64_t b = xcd_remap(blockIdx.x, (uint32_t)a.nblocks, 8u * (uint32_t)max(1, a.wpc / (kBlockRows / 64)));
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int64_t gw = b * (kBlockRows / 64) + wave;
@@ -1425,7 +1439,7 @@ __global__ __launch_bounds__(kBlockRows) void general_kernel(StepArgs a)
 
 // One time step of every lattice of an ensemble in wet-run waves: one wave
 // per entry of the ensemble's run table (sample, then the sample's own run
-// entry), each XCD taking one contiguous part of the table.  The samples'
+// entry), the XCDs taking the table in chunks (xcd_remap).  The samples'
 // perimeter segments run in batch_perimeter_kernel, as for the tiles.
 template <typename T>
 __global__ __launch_bounds__(kBlockRows) void batch_run_kernel(BatchArgs bt)
@@ -1471,7 +1485,7 @@ __global__ __launch_bounds__(kBlockRows) void batch_kernel(BatchArgs bt)
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if constexpr (DIRECT) {
-        // one wave per tile; each XCD takes one contiguous run of the table.
+        // one wave per tile; the XCDs take the table in chunks (xcd_remap).
         // The grid has ceil(ntiles / 4) work-groups (one when the table is
         // empty), counted here from ntiles rather than read from the
         // dispatch packet: one scalar round trip less before the tile.
