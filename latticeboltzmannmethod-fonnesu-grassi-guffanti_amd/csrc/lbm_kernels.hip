@@ -86,16 +86,19 @@ enum : int { BT_NONE = 0, BT_TOP, BT_RIGHT, BT_BOTTOM, BT_LEFT, BT_CORNER };
 // neighbouring columns at a time (their row segments share its L2) and all
 // eight move through memory together.  Blocks past the last whole group of
 // 8 chunks keep their order.  The row-segment step kernels use chunks of 8
-// columns, the others (wet runs, tables, ensembles, set-up) 256 blocks.
+// columns, the wet-run and tabled kernels 64 blocks, the others (ensembles,
+// set-up) 256 blocks.
 // Measured against one contiguous eighth of the tile order per XCD, one
 // process per variant, best launch shape per case: porous 8192^2 FP32 +5 %,
 // FP64 +5 %, FP16 +3 %; open 8192^2 FP32 +2 %, FP16 +5 %; 4096^2 cavity
 // FP32 +5 %, FP16 +6 %, FP64 +5 %; 512-lattice ensemble +3 %.  Chunks of
 // 256 blocks for the row segments too lost 3-4 % on the cavity's FP32 and
-// FP64 shapes with few blocks per column; 8 columns for the wet runs lost 1 %.
+// FP64 shapes with few blocks per column; for the wet runs 64 blocks beat
+// 256 by 1.3-1.9 % on the porous medium (all three formats).
 // Block counts stay below 2^31 (the geometry caps the 64-row segments at
 // 2^30), so the arithmetic is 32-bit.
 constexpr uint32_t kXcdChunk = 256;
+constexpr uint32_t kXcdChunkRuns = 64;
 
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb, uint32_t chunk = kXcdChunk)
 {
@@ -1343,7 +1346,7 @@ __global__ __launch_bounds__(kBlockRows) void run_kernel(StepArgs a)
     __shared__ int32_t rows[kBlockRows / 64][64];
     // every kernel argument the prologue needs, in one round trip
     asm volatile("" ::"s"(a.nblocks), "s"(a.nruns), "s"(a.runs), "s"(a.g.wet), "s"(a.g.sb), "s"(a.g.wl));
-    const int64_t b = xcd_remap(blockIdx.x, (uint32_t)a.nblocks);
+    const int64_t b = xcd_remap(blockIdx.x, (uint32_t)a.nblocks, kXcdChunkRuns);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t t = b * (kBlockRows / 64) + wave;
     if (t >= a.nruns) return;
@@ -1364,7 +1367,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlockRows) void tab_kernel(StepArgs a)
 {
     asm volatile("" ::"s"(a.nblocks), "s"(a.nruns), "s"(a.trun), "s"(a.tlane));  // one round trip
-    const int64_t b = xcd_remap(blockIdx.x, (uint32_t)a.nblocks);
+    const int64_t b = xcd_remap(blockIdx.x, (uint32_t)a.nblocks, kXcdChunkRuns);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t t = b * (kBlockRows / 64) + wave;
     if (t >= a.nruns) return;
