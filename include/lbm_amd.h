@@ -96,7 +96,13 @@ typedef enum lbm_options {
      * coherently (4096^2 cavity, 200k steps: velocity L2 error 0.44 rounded
      * to nearest, 2.8e-4 stochastic).  This bit selects round-to-nearest.
      * No reference counterpart. */
-    LBM_OPT_F16_ROUND_TO_NEAREST = 1
+    LBM_OPT_F16_ROUND_TO_NEAREST = 1,
+    /* Populations are stored blocked (the 9 directions of 64 consecutive
+     * wet nodes together) whenever a buffer of 9 * plane elements stays
+     * below 4 GiB, else in 9 planes.  This bit selects the planes for any
+     * size.  Results are the same in both layouts, bit for bit; the
+     * blocked one is faster.  No reference counterpart. */
+    LBM_OPT_PLANAR_LAYOUT = 2
 } lbm_options;
 
 /* Library version string. */
@@ -147,8 +153,11 @@ int lbm_set_partial_bounce_back(lbm_ctx *ctx, double tau, const double *b);
  * counterpart in the reference): the kernels address populations with
  * 32-bit byte offsets spanning three planes, so 3 * plane * element size
  * must stay below 4 GiB (plane = wet nodes + 1 rounded up to 64; about
- * 179M wet nodes for FP64, twice that for FP32), and wave indices are
- * 32-bit, so at most 2^30 64-row segments (ceil(NR / 64) * NC). */
+ * 179M wet nodes for FP64, twice that for FP32, four times for FP16), and
+ * wave indices are 32-bit, so at most 2^30 64-row segments
+ * (ceil(NR / 64) * NC).  Lattices up to a third of that (a whole buffer,
+ * 9 * plane * element size, below 4 GiB) are stored blocked unless the
+ * context was created with LBM_OPT_PLANAR_LAYOUT; larger ones in planes. */
 int lbm_set_geometry(lbm_ctx *ctx, const uint8_t *node_class);
 
 /* Number of nodes per class after set_geometry: out[0..5] indexed by

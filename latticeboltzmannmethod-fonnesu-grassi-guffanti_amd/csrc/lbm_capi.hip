@@ -61,6 +61,7 @@ StepArgs make_args(const lbm_ctx *c)
     a.collision = c->desc.collision;
     a.segs = (int32_t)c->segs;
     a.esize = (int32_t)c->esize;
+    a.block_mask = c->blocked ? ~63u : 0u;
     // The reference recomputes t = 1.0 / tau inside every call
     // (BGKCollision.hpp:140-141) and (1.0/tau_even) etc. inside every
     // expression (TRTCollision.hpp:155-188); IEEE division makes these the
@@ -474,7 +475,8 @@ int lbm_create(const lbm_desc *desc, lbm_ctx **out)
         return fail(LBM_ERR_INVALID, "unknown storage");
     for (int32_t p : {desc->inlet_policy, desc->outlet_policy})
         if (p < LBM_ZOUHE_VELOCITY || p > LBM_OPEN_NOOP) return fail(LBM_ERR_INVALID, "unknown open policy");
-    if (desc->options & ~(int32_t)LBM_OPT_F16_ROUND_TO_NEAREST) return fail(LBM_ERR_INVALID, "unknown option bits");
+    if (desc->options & ~(int32_t)(LBM_OPT_F16_ROUND_TO_NEAREST | LBM_OPT_PLANAR_LAYOUT))
+        return fail(LBM_ERR_INVALID, "unknown option bits");
     if (desc->collision == LBM_BGK && !(desc->tau > 0)) return fail(LBM_ERR_INVALID, "tau must be > 0");
     if (desc->collision == LBM_TRT && !(desc->tau_even > 0 && desc->tau_odd > 0))
         return fail(LBM_ERR_INVALID, "tau_even and tau_odd must be > 0");
@@ -723,13 +725,16 @@ int lbm_set_geometry(lbm_ctx *c, const uint8_t *cls)
     // dry lanes read their own slot, which past the last wet node of the
     // lattice is index `slots`.
     c->plane = ((slots + 1 + 63) / 64) * 64;
-    // The kernels address a population as a plane base pointer plus a
-    // 32-bit byte offset spanning up to three planes.
-    if ((uint64_t)3 * (uint64_t)c->plane * c->esize >= (1ull << 32)) {
+    // The kernels address a population as a direction base pointer plus a
+    // 32-bit byte offset: spanning up to three planes, or the whole buffer
+    // in the blocked layout, which lattices take when it fits.
+    const uint64_t fbytes_all = 9ull * (uint64_t)c->plane * c->esize;
+    if (fbytes_all / 3 >= (1ull << 32)) {
         c->nr = c->nc = 0;
-        return fail(LBM_ERR_UNSUPPORTED, "lattice with %lld wet nodes exceeds the 32-bit plane addressing "
+        return fail(LBM_ERR_UNSUPPORTED, "lattice with %lld wet nodes exceeds the 32-bit population addressing "
                     "(3 * compact slots * element size must stay below 4 GiB)", (long long)c->n_wet);
     }
+    c->blocked = !(c->desc.options & LBM_OPT_PLANAR_LAYOUT) && fbytes_all < (1ull << 32);
     // list order of the reference: row-major scan
     for (int64_t i = 0; i < nr; ++i)
         for (int64_t j = 0; j < nc; ++j) {

@@ -20,8 +20,13 @@ constexpr int kBlockRows = 256;
 
 // Device-side description of the lattice.
 //  * Populations are stored for wet nodes only, in column-major order (rows
-//    ascending within a column), 9 planes of `plane` elements: the node with
-//    compact index c, direction d, sits at d*plane + c.
+//    ascending within a column), 9 * `plane` elements per buffer (`plane` a
+//    multiple of 64), in one of two layouts:
+//      blocked: the 9 directions of compact nodes 64k .. 64k+63 together,
+//               (c, d) at 64*(9k + d) + c%64 -- every lattice whose buffer
+//               stays below 4 GiB, unless the context asks for planes;
+//      planes:  (c, d) at d*plane + c -- larger lattices (up to 3 planes
+//               below 4 GiB).
 //  * Wet mask: one bit per node, 64-bit words along rows, one guard word
 //    above and below every column and one guard column on each side, all
 //    zero: word (j, k) at wet[(j+1)*wl + (k+1)], bit i%64 of k = i/64.
@@ -48,11 +53,15 @@ struct StepArgs {
     Geometry g;
     const void *fin;      // state read by the step (F* of the previous step, or P)
     void *fout;           // F* written by the step
-    // Plane base pointers of fin / fout (plane d at fin + d * plane
-    // elements), passed explicitly so the kernels address every plane as
-    // a scalar base plus a 32-bit vector offset.
+    // Direction base pointers of fin / fout (direction d at fin + d * 64
+    // elements blocked, d * plane in planes), passed explicitly so the
+    // kernels address every population as a scalar base plus a 32-bit
+    // vector offset: compact node c of any direction at byte offset
+    // (c + 8 (c & block_mask)) * esize from the direction's base.
     const void *fin_plane[9];
     void *fout_plane[9];
+    uint32_t block_mask;    // ~63 blocked, 0 in planes
+    uint32_t two_planes;    // bytes from the base of direction d to that of d + 2
     const void *frozen;   // [9][E] populations with out-of-domain source, storage type
     const double *bc_prev;  // [3][E]: ux, uy, rho prescribed at the previous step
     const double *bc_cur;   // [3][E]: values prescribed at this step
@@ -109,7 +118,8 @@ constexpr int32_t kRunLaunch = LBM_LAUNCH_WET_RUNS;
 // The same runs with each lane's addressing read from a table.
 constexpr int32_t kRunTableLaunch = LBM_LAUNCH_RUN_TABLE;
 
-// Fills fin_plane / fout_plane from fin / fout.
+// Fills fin_plane / fout_plane and two_planes from fin / fout, the layout
+// (block_mask), the plane and the element size.
 void set_planes(StepArgs &a);
 
 // A batch step: per sample two complete argument sets, args[2 s + p] for

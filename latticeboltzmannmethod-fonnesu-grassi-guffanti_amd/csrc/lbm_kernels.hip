@@ -291,6 +291,22 @@ __device__ __forceinline__ void st(T *__restrict__ base, uint32_t byte_off, T v)
     *gaddr<T>((void *)base, byte_off) = v;
 }
 
+// Byte offset of compact node n from its direction's base pointer
+// (fin_plane / fout_plane): n in planes, n + 8 (n - n mod 64) blocked (the
+// 9 directions of 64 consecutive nodes together).  The layout is a scalar
+// mask, so both cost the same two instructions.  Both keep 64 consecutive
+// nodes from a 64-aligned index in one stretch per direction, so
+// whole-line stores stay whole lines.  Blocked, a wave's nine loads of a
+// 64-node range fall in one 9 * 64-element stretch instead of nine places
+// a plane apart; on MI355X that made the step kernels 4-7 % faster in FP64
+// and FP32 and 3-15 % in FP16 (8192^2 porous and open media, 4096^2
+// cavity), and the ensemble 8 %.
+template <typename T>
+__device__ __forceinline__ uint32_t slot_bytes(const StepArgs &a, uint32_t n)
+{
+    return (n + 8u * (n & a.block_mask)) * (uint32_t)sizeof(T);
+}
+
 // Streams the 9 populations of an interior wet node: one load per
 // direction, its address chosen between the neighbour (stream) and the
 // node's own opposite slot (bounce-back).  Opposite directions are two
@@ -299,9 +315,8 @@ __device__ __forceinline__ void st(T *__restrict__ base, uint32_t byte_off, T v)
 template <typename T, typename WetFn>
 __device__ __forceinline__ void gather_interior_by(const StepArgs &a, const Idx &x, WetFn wet_src, T *p)
 {
-    const uint32_t E = (uint32_t)sizeof(T);
-    const uint32_t twoP = 2u * (uint32_t)a.g.plane * E;
-    const uint32_t own = (uint32_t)x.src[0] * E;
+    const uint32_t twoP = a.two_planes;
+    const uint32_t own = slot_bytes<T>(a, (uint32_t)x.src[0]);
     p[0] = ld((const T *)a.fin_plane[0], own);
 #pragma unroll
     for (int d = 1; d < 9; ++d) {
@@ -309,7 +324,7 @@ __device__ __forceinline__ void gather_interior_by(const StepArgs &a, const Idx 
         const int o = kOpp[d];
         const bool low = d < o;  // d is the lower plane of its pair
         const T *base = (const T *)a.fin_plane[low ? d : o];
-        uint32_t nb = (uint32_t)x.src[d] * E + (low ? 0u : twoP);
+        uint32_t nb = slot_bytes<T>(a, (uint32_t)x.src[d]) + (low ? 0u : twoP);
         uint32_t ob = own + (low ? twoP : 0u);
         p[d] = ld(base, wet ? nb : ob);
     }
@@ -331,9 +346,8 @@ __device__ __forceinline__ uint32_t bit_insert(uint32_t s, uint32_t x, uint32_t 
 template <typename T>
 __device__ __forceinline__ void gather_interior(const StepArgs &a, const Idx &x, uint32_t m, T *p)
 {
-    const uint32_t E = (uint32_t)sizeof(T);
-    const uint32_t twoP = 2u * (uint32_t)a.g.plane * E;
-    const uint32_t own = (uint32_t)x.src[0] * E;
+    const uint32_t twoP = a.two_planes;
+    const uint32_t own = slot_bytes<T>(a, (uint32_t)x.src[0]);
     p[0] = ld((const T *)a.fin_plane[0], own);
 #pragma unroll
     for (int d = 1; d < 9; ++d) {
@@ -341,7 +355,7 @@ __device__ __forceinline__ void gather_interior(const StepArgs &a, const Idx &x,
         const int o = kOpp[d];
         const bool low = d < o;
         const T *base = (const T *)a.fin_plane[low ? d : o];
-        const uint32_t nb = (uint32_t)x.src[d] * E + (low ? 0u : twoP);
+        const uint32_t nb = slot_bytes<T>(a, (uint32_t)x.src[d]) + (low ? 0u : twoP);
         const uint32_t ob = own + (low ? twoP : 0u);
         p[d] = ld(base, bit_insert(sel, nb, ob));
     }
@@ -374,8 +388,8 @@ template <typename T>
 __device__ __forceinline__ void gather_edge(const StepArgs &a, const T *__restrict__ frozen, int i, int j,
                                             const Idx &x, int64_t e, uint32_t m, T *p)
 {
-    const uint32_t E = (uint32_t)sizeof(T);
-    p[0] = ld((const T *)a.fin_plane[0], (uint32_t)x.src[0] * E);
+    const uint32_t own = slot_bytes<T>(a, (uint32_t)x.src[0]);
+    p[0] = ld((const T *)a.fin_plane[0], own);
 #pragma unroll
     for (int d = 1; d < 9; ++d) {
         int si = i - kOffI[d], sj = j - kOffJ[d];
@@ -384,8 +398,8 @@ __device__ __forceinline__ void gather_edge(const StepArgs &a, const T *__restri
             p[d] = frozen[d * a.g.E + e];
         } else {
             bool wet = (m >> d) & 1;
-            p[d] = wet ? ld((const T *)a.fin_plane[d], (uint32_t)x.src[d] * E)
-                       : ld((const T *)a.fin_plane[kOpp[d]], (uint32_t)x.src[0] * E);
+            p[d] = wet ? ld((const T *)a.fin_plane[d], slot_bytes<T>(a, (uint32_t)x.src[d]))
+                       : ld((const T *)a.fin_plane[kOpp[d]], own);
         }
     }
 }
@@ -398,7 +412,7 @@ __device__ __forceinline__ void gather_edge(const StepArgs &a, const T *__restri
 template <typename T>
 __device__ __forceinline__ void store9(const StepArgs &a, int64_t n, const T *o)
 {
-    uint32_t b = (uint32_t)n * (uint32_t)sizeof(T);
+    uint32_t b = slot_bytes<T>(a, (uint32_t)n);
     asm volatile("" : "+v"(b));
 #pragma unroll
     for (int d = 0; d < 9; ++d) st((T *)a.fout_plane[d], b, o[d]);
@@ -413,7 +427,7 @@ __device__ __forceinline__ void store9(const StepArgs &a, int64_t n, const T *o)
 template <typename T>
 __device__ __forceinline__ void store9_stream(const StepArgs &a, int64_t n, const T *o)
 {
-    uint32_t b = (uint32_t)n * (uint32_t)sizeof(T);
+    uint32_t b = slot_bytes<T>(a, (uint32_t)n);
     asm volatile("" : "+v"(b)::"memory");
 #pragma unroll
     for (int d = 0; d < 9; ++d) __builtin_nontemporal_store(o[d], gaddr<T>(a.fout_plane[d], b));
@@ -580,7 +594,7 @@ __device__ __forceinline__ int load_p64(const StepArgs &a, int i, int j, const I
             if (a.psbb) psbb_gather(a, x.src[0], p);
         } else {
 #pragma unroll
-            for (int d = 0; d < 9; ++d) p[d] = ld((const double *)a.fin_plane[d], (uint32_t)x.src[0] * 8u);
+            for (int d = 0; d < 9; ++d) p[d] = ld((const double *)a.fin_plane[d], slot_bytes<double>(a, x.src[0]));
         }
         return -1;
     }
@@ -590,7 +604,7 @@ __device__ __forceinline__ int load_p64(const StepArgs &a, int i, int j, const I
         if (a.psbb) psbb_gather(a, x.src[0], p);
     } else {
 #pragma unroll
-        for (int d = 0; d < 9; ++d) p[d] = ld((const double *)a.fin_plane[d], (uint32_t)x.src[0] * 8u);
+        for (int d = 0; d < 9; ++d) p[d] = ld((const double *)a.fin_plane[d], slot_bytes<double>(a, x.src[0]));
     }
     int kind = a.g.edge_kind[e];
     if (kind != kEdgePlain) {
@@ -774,7 +788,7 @@ __device__ __forceinline__ int load_g(const StepArgs &a, int i, int j, const Idx
 {
     T raw[9];
     int policy = -1;
-    const uint32_t own = (uint32_t)x.src[0] * (uint32_t)sizeof(T);
+    const uint32_t own = slot_bytes<T>(a, (uint32_t)x.src[0]);
     if (!edge) {
         if (GATHER) gather_interior(a, x, m, raw);
         else
@@ -1141,7 +1155,7 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
             const int i = (int)row0 + 64 * r + lane;
             collide_shifted<false>(a, raw[r], -1, 0, i, j, out[r]);
         }
-        uint32_t b0 = (uint32_t)own[0] * 4u, b1 = (uint32_t)own[1] * 4u;
+        uint32_t b0 = slot_bytes<T>(a, (uint32_t)own[0]), b1 = slot_bytes<T>(a, (uint32_t)own[1]);
         if ((lines & 3u) == 3u) {
             asm volatile("" : "+v"(b0), "+v"(b1)::"memory");
 #pragma unroll
@@ -1501,7 +1515,7 @@ __global__ __launch_bounds__(kBlockRows) void batch_kernel(BatchArgs bt)
         // the plane pointers are loaded with the sample's other arguments,
         // not after its mask words arrive
         asm volatile("" ::"s"(a.fin_plane[0]), "s"(a.fin_plane[1]), "s"(a.fin_plane[2]), "s"(a.fin_plane[5]),
-                     "s"(a.fin_plane[6]), "s"(a.g.plane));
+                     "s"(a.fin_plane[6]), "s"(a.block_mask), "s"(a.two_planes));
         step_wave<T, GATHER, R>(a, tr.j, tr.row0, lane);
         return;
     }
@@ -1675,7 +1689,7 @@ __global__ __launch_bounds__(kBlockRows) void upload(StepArgs a, const double *s
     if (!((h.mid[1] >> t.lane) & 1)) return;
     const int i = t.i, j = t.j;
     const int64_t N = (int64_t)a.g.nr * a.g.nc, q = (int64_t)j * a.g.nr + i;
-    const uint32_t off = (uint32_t)lane_idx(h, t.lane).src[0] * (uint32_t)sizeof(T);
+    const uint32_t off = slot_bytes<T>(a, (uint32_t)lane_idx(h, t.lane).src[0]);
     const bool edge = (i == 0) | (i == a.g.nr - 1) | (j == 0) | (j == a.g.nc - 1);
     const int64_t e = edge ? edge_slot(i, j, a.g.nr, a.g.nc) : 0;
 #pragma unroll
@@ -1705,7 +1719,7 @@ __global__ __launch_bounds__(kBlockRows) void init_eq(StepArgs a, T *frozen, dou
     }
     const Nbhd h = load_nbhd(a.g, t.j, t.i0);
     if (!((h.mid[1] >> t.lane) & 1)) return;
-    const uint32_t off = (uint32_t)lane_idx(h, t.lane).src[0] * (uint32_t)sizeof(T);
+    const uint32_t off = slot_bytes<T>(a, (uint32_t)lane_idx(h, t.lane).src[0]);
     const bool edge = (i == 0) | (i == a.g.nr - 1) | (j == 0) | (j == a.g.nc - 1);
     const int64_t e = edge ? edge_slot(i, j, a.g.nr, a.g.nc) : 0;
     double e9[9];
@@ -1742,8 +1756,7 @@ __device__ __forceinline__ int64_t cidx_of(const Geometry &g, int64_t i, int64_t
 template <typename T>
 __device__ __forceinline__ double fstar_at(const StepArgs &a, int64_t i, int64_t j, int d)
 {
-    const T *F = (const T *)a.fin_plane[d];
-    T v = F[cidx_of(a.g, i, j)];
+    const T v = ld((const T *)a.fin_plane[d], slot_bytes<T>(a, (uint32_t)cidx_of(a.g, i, j)));
     if constexpr (sizeof(T) == 8) return (double)v;
     else return (double)to_g<T>(v, a.inv_scale) + kW[d];
 }
@@ -1944,7 +1957,8 @@ void launch_batch_partials_r(bool gather, int32_t R, const BatchArgs &b, double 
 
 void set_planes(StepArgs &a)
 {
-    const size_t stride = (size_t)a.g.plane * (size_t)a.esize;
+    const size_t stride = (size_t)(a.block_mask ? 64 : a.g.plane) * (size_t)a.esize;
+    a.two_planes = (uint32_t)(2 * stride);
     for (int d = 0; d < 9; ++d) {
         a.fin_plane[d] = a.fin ? (const char *)a.fin + (size_t)d * stride : nullptr;
         a.fout_plane[d] = a.fout ? (char *)a.fout + (size_t)d * stride : nullptr;

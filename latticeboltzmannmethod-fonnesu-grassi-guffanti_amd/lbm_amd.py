@@ -29,6 +29,7 @@ ZOUHE_VELOCITY, ZOUHE_PRESSURE, OPEN_NOOP = 0, 1, 2
 F64, F32_SHIFTED, F16_SHIFTED = 0, 1, 2
 STEP_MACRO, STEP_CHECK, STEP_NO_GRAPH = 1, 2, 4
 OPT_F16_ROUND_TO_NEAREST = 1
+OPT_PLANAR_LAYOUT = 2
 # lbm_set_launch shapes: one wave per run of up to 64 consecutive wet rows;
 # the same runs with each lane's addressing read from a table
 LAUNCH_WET_RUNS = 64

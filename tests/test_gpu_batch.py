@@ -3,11 +3,11 @@ persistent kernel per step over a tile table, with HIP-graph replay of the
 step loop and an autotuned tile shape.
 
 Bar: every sample of an ensemble ends bit for bit where the same lattice
-stepped alone ends (FP64 and FP32 storage), graph replays equal eager
-launches bit for bit across calls with other work in between, the FP64
-ensemble matches the CPU oracle to 1e-12, and the per-sample FP64
-reductions (and the Darcy permeability computed from them) agree with the
-single-lattice reduction to 1e-12 relative."""
+stepped alone ends (every storage format, samples in either population
+layout), graph replays equal eager launches bit for bit across calls with
+other work in between, the FP64 ensemble matches the CPU oracle to 1e-12,
+and the per-sample FP64 reductions (and the Darcy permeability computed
+from them) agree with the single-lattice reduction to 1e-12 relative."""
 import sys
 
 import numpy as np
@@ -39,12 +39,15 @@ def run_alone(wl, storage, schedule):
     return p
 
 
+@pytest.mark.parametrize("layouts", ["blocked", "mixed"])
 @pytest.mark.parametrize("launch", ["tiles", "wet-runs"])
 @pytest.mark.parametrize("storage", [L.F64, L.F32_SHIFTED, L.F16_SHIFTED])
-def test_ensemble_equals_lattices_alone(storage, launch):
+def test_ensemble_equals_lattices_alone(storage, launch, layouts):
+    """mixed: every other sample stores its populations in planes."""
     wls = small_ensemble()
-    say(f"case ensemble of {len(wls)} ({', '.join(w.name for w in wls)}), storage {storage}, {launch}")
-    lats = [w.make_lattice(storage) for w in wls]
+    say(f"case ensemble of {len(wls)} ({', '.join(w.name for w in wls)}), storage {storage}, {launch}, {layouts}")
+    lats = [w.make_lattice(storage, options=L.OPT_PLANAR_LAYOUT if layouts == "mixed" and k % 2 else 0)
+            for k, w in enumerate(wls)]
     schedule = [1, 40, 7, 64]          # first step, graph + eager remainder, eager, graph replays
     with L.Batch(lats) as b:
         if launch == "wet-runs":

@@ -1,14 +1,22 @@
-"""The largest lattices the 32-bit plane addressing admits.  The kernels
-address a population as a plane base plus a 32-bit byte offset spanning up
-to three planes, so a context accepts a lattice only while
+"""The largest lattices the 32-bit population addressing admits.  The
+kernels address a population as a direction base plus a 32-bit byte offset
+spanning up to three planes, so a context accepts a lattice only while
 3 * plane * element size < 2^32 (plane = compact slots rounded up to 64,
-with one spare slot).  Here, just below and just above that bound: the
-largest FP64 lattice keeps a closed box at rest exactly (mass and momentum
-from the FP64 reductions), in both launch shapes; one column more is
-rejected with LBM_ERR_UNSUPPORTED; FP32 storage accepts that size.
+with one spare slot); lattices whose whole buffer, 9 * plane * element
+size, stays below 2^32 are stored blocked, larger ones in planes.  Here,
+just below and just above those bounds: the largest FP64 lattice keeps a
+closed box at rest exactly (mass and momentum from the FP64 reductions), in
+both launch shapes; one column more is rejected with LBM_ERR_UNSUPPORTED;
+FP32 storage accepts that size.  The momentum a lid puts into the largest
+lattices of each layout (FP64 and FP32 in planes, the largest blocked FP32
+and FP16 ones) after a few steps is the linear function of the width that
+two narrow cavities fix: every column under the lid away from the side
+walls computes the same values, so a population read from a wrong slot
+anywhere along the lid, up to the highest offsets, would show.
 
 A closed box of NR - 2 = 13312 = 208 * 64 fluid rows fills whole 64-slot
-groups, so its slot count is 1 + (NC - 2) * 13312."""
+groups, so its slot count is 1 + (NC - 2) * 13312; so does a lid cavity of
+NR_LID = NR - 1 rows (the lid row and NR_LID - 2 fluid rows per column)."""
 import sys
 
 import numpy as np
@@ -21,6 +29,8 @@ pytestmark = pytest.mark.gpu
 NR = 13314
 NC_FIT = 13445      # 3 * plane * 8 = 4,294,878,720 bytes < 2^32
 NC_OVER = 13446     # 3 * plane * 8 = 4,295,198,208 bytes > 2^32
+NC_F32_BLOCKED = 8964   # 9 * plane * 4 = 4,294,879,488 bytes < 2^32 (8965: 4,295,358,720)
+NR_LID = NR - 1
 
 
 def say(msg):
@@ -35,10 +45,10 @@ def closed_box(nr, nc):
     return cls
 
 
-def plane_bytes(nr, nc, esize):
+def plane_bytes(nr, nc, esize, planes=3):
     slots = 1 + (nc - 2) * (nr - 2)
     plane = (slots + 1 + 63) // 64 * 64
-    return 3 * plane * esize
+    return planes * plane * esize
 
 
 def test_largest_fp64_lattice_keeps_the_rest_state():
@@ -81,3 +91,42 @@ def test_one_column_more_is_rejected_in_fp64_and_accepted_in_fp32():
         assert red[1] == 0.0 and red[2] == 0.0
     finally:
         lat.close()
+
+
+def lid_momentum(nr, nc, storage, options, steps):
+    """x and y momentum of the fluid nodes of an nr x nc lid-driven cavity
+    (BGK, tau 0.8, lid u = (0.1, 0)) after `steps` steps."""
+    cls = L.geometry_cavity(nr, nc)
+    lat = L.Lattice(cls, collision=L.BGK, tau=0.8, storage=storage, inlet_policy=L.ZOUHE_VELOCITY,
+                    outlet_policy=L.OPEN_NOOP, options=options)
+    try:
+        lat.set_open_values(0, ux=0.1, uy=0.0)
+        lat.init_equilibrium()
+        lat.step(steps, L.STEP_CHECK)
+        red = lat.reduce()
+        return red[1], red[2]
+    finally:
+        lat.close()
+
+
+@pytest.mark.parametrize("storage,options,nc,layout", [
+    (L.F64, 0, NC_FIT, "planes"),
+    (L.F32_SHIFTED, 0, NC_FIT, "planes"),
+    (L.F32_SHIFTED, 0, NC_F32_BLOCKED, "blocked"),
+    (L.F16_SHIFTED, L.OPT_F16_ROUND_TO_NEAREST, NC_FIT, "blocked"),
+])
+def test_lid_momentum_of_the_largest_lattices_is_linear_in_the_width(storage, options, nc, layout):
+    esize = {L.F64: 8, L.F32_SHIFTED: 4, L.F16_SHIFTED: 2}[storage]
+    assert (plane_bytes(NR, nc, esize, 9) < 2 ** 32) == (layout == "blocked")
+    assert plane_bytes(NR, nc, esize) < 2 ** 32
+    steps = 4
+    say(f"case lid cavity {NR_LID}x{nc}, storage {storage}, options {options} ({layout}), {steps} steps")
+    n1, n2 = 64, 96
+    m1 = lid_momentum(64, n1, storage, options, steps)
+    m2 = lid_momentum(64, n2, storage, options, steps)
+    m = lid_momentum(NR_LID, nc, storage, options, steps)
+    for k, name in ((0, "x"), (1, "y")):
+        expect = m1[k] + (m2[k] - m1[k]) * (nc - n1) / (n2 - n1)
+        say(f"  momentum {name}: {m[k]:.15e}, from the narrow cavities {expect:.15e}")
+        assert abs(m[k] - expect) <= 1e-10 * abs(m1[0] + (m2[0] - m1[0]) * (nc - n1) / (n2 - n1))
+    assert abs(m[0]) > 1e-3 * nc
