@@ -192,7 +192,8 @@ def time_storage(wl, name, steps, warmup, ev):
     achieved = alg / per_step_s / 1e9
     return lat, dict(storage=name, mlups=mlups, mlups_all_nodes=wl.nr * wl.nc / per_step_s / 1e6,
                      ms_per_step=ms / steps, wall_s=wall, alg_bytes_per_step=alg, achieved_gbs=achieved,
-                     frac=achieved / HBM_PEAK_GBS, segments=segments,
+                     frac=achieved / HBM_PEAK_GBS, segments=segments, occupancy=lat.occupancy(),
+                     layout="blocked" if lat.layout() == L.LAYOUT_BLOCKED else "planes",
                      traffic_bytes_per_step=measured_traffic(f"{wl.name}/{name}/R{segments}"))
 
 
@@ -293,7 +294,8 @@ def bench_cavity_trt(args, ev):
         results[s] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
         lats[s] = lat
         log(f"  cavity_trt4096 {s}: {r['mlups']:.1f} MLUPS, {r['ms_per_step']:.4f} ms/step, "
-            f"{r['achieved_gbs']:.0f} GB/s = {100 * r['frac']:.1f}% of 8 TB/s (launch {r['segments']})")
+            f"{r['achieved_gbs']:.0f} GB/s = {100 * r['frac']:.1f}% of 8 TB/s "
+            f"(launch {r['segments']}, occupancy {r['occupancy'] or 'full'})")
     errors = None
     if args.c4_error_steps > 0:
         t1 = time.perf_counter()
@@ -420,7 +422,9 @@ def main():
         lat, r = time_storage(wl, s, args.steps, args.warmup, ev)
         r["frac_of_copy"] = r["achieved_gbs"] / ceiling["gbs"]
         if patterns:
-            r["frac_of_pattern"] = r["achieved_gbs"] / patterns[s[:3]]["ab_pull"]
+            # the dense pattern of the layout the lattice got
+            pat = "blocked_pull" if lat.layout() == L.LAYOUT_BLOCKED else "ab_pull"
+            r["frac_of_pattern"] = r["achieved_gbs"] / patterns[s[:3]][pat]
         results[s] = r
         log(f"  {r['storage']}: {r['mlups']:.1f} MLUPS, {r['ms_per_step']:.4f} ms/step, "
             f"{r['achieved_gbs']:.0f} GB/s = {100 * r['frac']:.1f}% of 8 TB/s, "

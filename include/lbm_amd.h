@@ -160,6 +160,12 @@ int lbm_set_partial_bounce_back(lbm_ctx *ctx, double tau, const double *b);
  * context was created with LBM_OPT_PLANAR_LAYOUT; larger ones in planes. */
 int lbm_set_geometry(lbm_ctx *ctx, const uint8_t *node_class);
 
+/* Population layout the geometry got (LBM_ERR_STATE before any):
+ * LBM_LAYOUT_BLOCKED or LBM_LAYOUT_PLANES (see LBM_OPT_PLANAR_LAYOUT).  No
+ * counterpart in the reference. */
+enum { LBM_LAYOUT_BLOCKED = 0, LBM_LAYOUT_PLANES = 1 };
+int lbm_get_layout(lbm_ctx *ctx, int32_t *layout);
+
 /* Number of nodes per class after set_geometry: out[0..5] indexed by
  * lbm_node_class, out[6] = wet obstacle nodes (directions mask != 0). */
 int lbm_node_counts(lbm_ctx *ctx, int64_t out[7]);
@@ -300,12 +306,22 @@ enum { LBM_LAUNCH_WET_RUNS = 64, LBM_LAUNCH_RUN_TABLE = 65 };
 int lbm_set_launch(lbm_ctx *ctx, int32_t segments);
 int lbm_get_launch(lbm_ctx *ctx, int32_t *segments);
 
+/* Occupancy of the step kernels: at most `groups_per_cu` work-groups (of
+ * four waves) per compute unit, 2 to 7, or 0 for as many as their
+ * registers allow (8 for most shapes).  Fewer waves in flight can move more
+ * bytes per second (MI355X, FP64 and FP32 storage).  Results are the same
+ * bits at every occupancy.  No counterpart in the reference. */
+int lbm_set_occupancy(lbm_ctx *ctx, int32_t groups_per_cu);
+int lbm_get_occupancy(lbm_ctx *ctx, int32_t *groups_per_cu);
+
 /* Times each launch shape (1, 2, 4, 8 segments, wet runs, tabled wet runs)
+ * at each occupancy (no limit, 7 down to 3 work-groups per compute unit)
  * over `steps` launches (every launch rewrites the spare buffer from the
- * current state, so the state does not change) and keeps the fastest.
- * Reports the choice and its time per step (outputs may be NULL).
- * LBM_ERR_STATE for a context that belongs to an ensemble (the ensemble
- * has its own, lbm_batch_autotune).  No counterpart in the reference. */
+ * current state, so the state does not change) and keeps the fastest
+ * pair.  Reports the shape and its time per step (outputs may be NULL);
+ * lbm_get_occupancy reports the occupancy.  LBM_ERR_STATE for a context
+ * that belongs to an ensemble (the ensemble has its own,
+ * lbm_batch_autotune).  No counterpart in the reference. */
 int lbm_autotune(lbm_ctx *ctx, int32_t steps, int32_t *segments, double *ms_per_step);
 
 /* ------------------------------------------------------------------ */

@@ -62,6 +62,7 @@ StepArgs make_args(const lbm_ctx *c)
     a.segs = (int32_t)c->segs;
     a.esize = (int32_t)c->esize;
     a.block_mask = c->blocked ? ~63u : 0u;
+    a.launch_lds = occupancy_lds(c->occupancy);
     // The reference recomputes t = 1.0 / tau inside every call
     // (BGKCollision.hpp:140-141) and (1.0/tau_even) etc. inside every
     // expression (TRTCollision.hpp:155-188); IEEE division makes these the
@@ -1058,6 +1059,31 @@ int lbm_set_launch(lbm_ctx *c, int32_t segments)
     return LBM_OK;
 }
 
+int lbm_get_layout(lbm_ctx *c, int32_t *layout)
+{
+    if (!c || !layout) return fail(LBM_ERR_INVALID, "null argument");
+    if (!c->has_geom) return fail(LBM_ERR_STATE, "no geometry");
+    *layout = c->blocked ? LBM_LAYOUT_BLOCKED : LBM_LAYOUT_PLANES;
+    return LBM_OK;
+}
+
+int lbm_set_occupancy(lbm_ctx *c, int32_t groups_per_cu)
+{
+    if (!c) return fail(LBM_ERR_INVALID, "null context");
+    if (groups_per_cu != 0 && (groups_per_cu < 2 || groups_per_cu > 7))
+        return fail(LBM_ERR_INVALID, "occupancy must be 0 (no limit) or 2..7 work-groups per compute unit");
+    c->occupancy = groups_per_cu;
+    drop_graphs(c);
+    return LBM_OK;
+}
+
+int lbm_get_occupancy(lbm_ctx *c, int32_t *groups_per_cu)
+{
+    if (!c || !groups_per_cu) return fail(LBM_ERR_INVALID, "null argument");
+    *groups_per_cu = c->occupancy;
+    return LBM_OK;
+}
+
 int lbm_get_launch(lbm_ctx *c, int32_t *segments)
 {
     if (!c || !segments) return fail(LBM_ERR_INVALID, "null argument");
@@ -1082,7 +1108,7 @@ int lbm_autotune(lbm_ctx *c, int32_t steps, int32_t *segments, double *ms_per_st
     HIP_TRY(hipEventCreate(&t0));
     hipError_t err = hipEventCreate(&t1);
     float best = -1.f;
-    int32_t best_R = default_segments(c->storage);
+    int32_t best_R = default_segments(c->storage), best_occ = 0;
     // Each launch recomputes the spare buffer from the current one, so the
     // state is the same after tuning.
     StepArgs a = make_args(c);
@@ -1090,23 +1116,29 @@ int lbm_autotune(lbm_ctx *c, int32_t steps, int32_t *segments, double *ms_per_st
     a.fin = c->d_F[c->cur];
     a.fout = c->d_F[1 - c->cur];
     a.macro = nullptr;
-    for (int32_t R : {1, 2, 4, 8, kRunLaunch, kRunTableLaunch}) {
-        if (err == hipSuccess) err = launch_step(c->storage, true, a, c->stream, R);
-        if (err == hipSuccess) err = hipEventRecord(t0, c->stream);
-        for (int32_t k = 0; k < steps && err == hipSuccess; ++k) err = launch_step(c->storage, true, a, c->stream, R);
-        if (err == hipSuccess) err = hipEventRecord(t1, c->stream);
-        if (err == hipSuccess) err = hipEventSynchronize(t1);
-        float ms = 0.f;
-        if (err == hipSuccess) err = hipEventElapsedTime(&ms, t0, t1);
-        if (err == hipSuccess && (best < 0 || ms < best)) {
-            best = ms;
-            best_R = R;
+    for (int32_t occ : {0, 7, 6, 5, 4, 3}) {
+        a.launch_lds = occupancy_lds(occ);
+        for (int32_t R : {1, 2, 4, 8, kRunLaunch, kRunTableLaunch}) {
+            if (err == hipSuccess) err = launch_step(c->storage, true, a, c->stream, R);
+            if (err == hipSuccess) err = hipEventRecord(t0, c->stream);
+            for (int32_t k = 0; k < steps && err == hipSuccess; ++k)
+                err = launch_step(c->storage, true, a, c->stream, R);
+            if (err == hipSuccess) err = hipEventRecord(t1, c->stream);
+            if (err == hipSuccess) err = hipEventSynchronize(t1);
+            float ms = 0.f;
+            if (err == hipSuccess) err = hipEventElapsedTime(&ms, t0, t1);
+            if (err == hipSuccess && (best < 0 || ms < best)) {
+                best = ms;
+                best_R = R;
+                best_occ = occ;
+            }
         }
     }
     (void)hipEventDestroy(t0);
     if (t1) (void)hipEventDestroy(t1);
     if (err != hipSuccess) return fail(LBM_ERR_HIP, "autotune: %s", hipGetErrorString(err));
     c->segments = best_R;
+    c->occupancy = best_occ;
     drop_graphs(c);
     if (segments) *segments = best_R;
     if (ms_per_step) *ms_per_step = best / steps;

@@ -62,6 +62,7 @@ struct StepArgs {
     void *fout_plane[9];
     uint32_t block_mask;    // ~63 blocked, 0 in planes
     uint32_t two_planes;    // bytes from the base of direction d to that of d + 2
+    uint32_t launch_lds;    // host side: dynamic LDS per work-group of the step launches (occupancy_lds)
     const void *frozen;   // [9][E] populations with out-of-domain source, storage type
     const double *bc_prev;  // [3][E]: ux, uy, rho prescribed at the previous step
     const double *bc_cur;   // [3][E]: values prescribed at this step
@@ -121,6 +122,14 @@ constexpr int32_t kRunTableLaunch = LBM_LAUNCH_RUN_TABLE;
 // Fills fin_plane / fout_plane and two_planes from fin / fout, the layout
 // (block_mask), the plane and the element size.
 void set_planes(StepArgs &a);
+
+// Dynamic LDS per work-group that leaves room for exactly `groups_per_cu`
+// work-groups (2..7) in a compute unit's 160 KiB, or 0 (no limit).  The
+// step kernels' own LDS (at most 1 KiB) does not change the count.
+constexpr uint32_t occupancy_lds(int32_t groups_per_cu)
+{
+    return groups_per_cu >= 2 && groups_per_cu <= 7 ? 163840u / (uint32_t)(groups_per_cu + 1) + 64u : 0u;
+}
 
 // A batch step: per sample two complete argument sets, args[2 s + p] for
 // buffer parity p (p = 0: input in the sample's current buffer).

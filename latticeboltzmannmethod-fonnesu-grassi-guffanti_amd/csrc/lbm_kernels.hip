@@ -1829,7 +1829,7 @@ hipError_t launch(K kernel, int64_t nb, hipStream_t s, StepArgs a)
 {
     a.nblocks = nb;
     set_planes(a);
-    hipLaunchKernelGGL(kernel, dim3((unsigned)nb), dim3(kBlockRows), 0, s, a);
+    hipLaunchKernelGGL(kernel, dim3((unsigned)nb), dim3(kBlockRows), a.launch_lds, s, a);
     return hipGetLastError();
 }
 

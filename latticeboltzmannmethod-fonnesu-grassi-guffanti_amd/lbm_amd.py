@@ -30,6 +30,7 @@ F64, F32_SHIFTED, F16_SHIFTED = 0, 1, 2
 STEP_MACRO, STEP_CHECK, STEP_NO_GRAPH = 1, 2, 4
 OPT_F16_ROUND_TO_NEAREST = 1
 OPT_PLANAR_LAYOUT = 2
+LAYOUT_BLOCKED, LAYOUT_PLANES = 0, 1
 # lbm_set_launch shapes: one wave per run of up to 64 consecutive wet rows;
 # the same runs with each lane's addressing read from a table
 LAUNCH_WET_RUNS = 64
@@ -46,7 +47,9 @@ EXPORTED = [
     "lbm_host_free", "lbm_lift_drag", "lbm_lift_drag_async", "lbm_reduce", "lbm_reduce_device", "lbm_synchronize",
     "lbm_get_stream",
     "lbm_get_scale", "lbm_geometry_cavity", "lbm_geometry_channel", "lbm_geometry_add_disc",
-    "lbm_geometry_porous", "lbm_set_launch", "lbm_get_launch", "lbm_autotune", "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_set_launch", "lbm_batch_get_launch",
+    "lbm_geometry_porous", "lbm_set_launch", "lbm_get_launch", "lbm_set_occupancy", "lbm_get_occupancy",
+    "lbm_get_layout",
+    "lbm_autotune", "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_set_launch", "lbm_batch_get_launch",
     "lbm_batch_autotune", "lbm_batch_step", "lbm_batch_reduce", "lbm_batch_info", "lbm_batch_get_stream",
     "lbm_batch_synchronize", "lbm_copy_bandwidth", "lbm_pattern_bandwidth",
 ]
@@ -121,6 +124,9 @@ def lib():
                                     i64p],
             "lbm_set_launch": [vp, C.c_int32],
             "lbm_get_launch": [vp, i32p],
+            "lbm_set_occupancy": [vp, C.c_int32],
+            "lbm_get_layout": [vp, i32p],
+            "lbm_get_occupancy": [vp, i32p],
             "lbm_autotune": [vp, C.c_int32, i32p, dp],
             "lbm_batch_create": [C.POINTER(vp), C.c_int64, C.POINTER(vp)],
             "lbm_batch_destroy": [vp],
@@ -373,7 +379,24 @@ class Lattice:
         _check(lib().lbm_get_launch(self._h, C.byref(r)))
         return r.value
 
+    def layout(self):
+        """LAYOUT_BLOCKED or LAYOUT_PLANES (lbm_get_layout)."""
+        r = C.c_int32(0)
+        _check(lib().lbm_get_layout(self._h, C.byref(r)))
+        return r.value
+
+    def set_occupancy(self, groups_per_cu):
+        """Work-groups per compute unit of the step kernels (2..7), 0 = no limit."""
+        _check(lib().lbm_set_occupancy(self._h, groups_per_cu))
+
+    def occupancy(self):
+        r = C.c_int32(0)
+        _check(lib().lbm_get_occupancy(self._h, C.byref(r)))
+        return r.value
+
     def autotune(self, steps=5):
+        """Fastest launch shape and occupancy (lbm_autotune); returns the shape
+        and its ms per step (occupancy() gives the occupancy)."""
         r, ms = C.c_int32(0), C.c_double(0)
         _check(lib().lbm_autotune(self._h, steps, C.byref(r), C.byref(ms)))
         return r.value, ms.value

@@ -37,6 +37,7 @@ def test_planar_and_blocked_layouts_agree_bit_for_bit(storage, options):
         blocked = wl.make_lattice(storage, options=options)
         planar = wl.make_lattice(storage, options=options | L.OPT_PLANAR_LAYOUT)
         try:
+            assert blocked.layout() == L.LAYOUT_BLOCKED and planar.layout() == L.LAYOUT_PLANES
             for shape in SHAPES:
                 for lat in (blocked, planar):
                     lat.init_equilibrium()

@@ -59,6 +59,7 @@ def test_largest_fp64_lattice_keeps_the_rest_state():
         f"3 planes = {plane_bytes(NR, NC_FIT, 8)} bytes")
     lat = L.Lattice(cls, collision=L.BGK, tau=0.8, storage=L.F64)
     try:
+        assert lat.layout() == L.LAYOUT_PLANES
         lat.init_equilibrium()
         m0 = lat.reduce()
         for shape in (1, L.LAUNCH_WET_RUNS):
@@ -82,6 +83,7 @@ def test_one_column_more_is_rejected_in_fp64_and_accepted_in_fp32():
     assert e.value.code == 5
     lat = L.Lattice(cls, collision=L.BGK, tau=0.8, storage=L.F32_SHIFTED)
     try:
+        assert lat.layout() == L.LAYOUT_PLANES
         lat.init_equilibrium()
         lat.step(3, L.STEP_CHECK)
         red = lat.reduce()
@@ -93,13 +95,16 @@ def test_one_column_more_is_rejected_in_fp64_and_accepted_in_fp32():
         lat.close()
 
 
-def lid_momentum(nr, nc, storage, options, steps):
+def lid_momentum(nr, nc, storage, options, steps, layout=None):
     """x and y momentum of the fluid nodes of an nr x nc lid-driven cavity
-    (BGK, tau 0.8, lid u = (0.1, 0)) after `steps` steps."""
+    (BGK, tau 0.8, lid u = (0.1, 0)) after `steps` steps; checks the layout
+    the lattice got when one is given."""
     cls = L.geometry_cavity(nr, nc)
     lat = L.Lattice(cls, collision=L.BGK, tau=0.8, storage=storage, inlet_policy=L.ZOUHE_VELOCITY,
                     outlet_policy=L.OPEN_NOOP, options=options)
     try:
+        if layout is not None:
+            assert lat.layout() == {"blocked": L.LAYOUT_BLOCKED, "planes": L.LAYOUT_PLANES}[layout]
         lat.set_open_values(0, ux=0.1, uy=0.0)
         lat.init_equilibrium()
         lat.step(steps, L.STEP_CHECK)
@@ -124,7 +129,7 @@ def test_lid_momentum_of_the_largest_lattices_is_linear_in_the_width(storage, op
     n1, n2 = 64, 96
     m1 = lid_momentum(64, n1, storage, options, steps)
     m2 = lid_momentum(64, n2, storage, options, steps)
-    m = lid_momentum(NR_LID, nc, storage, options, steps)
+    m = lid_momentum(NR_LID, nc, storage, options, steps, layout)
     for k, name in ((0, "x"), (1, "y")):
         expect = m1[k] + (m2[k] - m1[k]) * (nc - n1) / (n2 - n1)
         say(f"  momentum {name}: {m[k]:.15e}, from the narrow cavities {expect:.15e}")

@@ -216,26 +216,43 @@ def test_graph_replay_equals_eager_steps():
 
 
 def test_autotune_keeps_state_and_results():
-    """The tile-shape autotuner must not change the state, and every tile
-    shape must give the same bits."""
+    """The autotuner (launch shape and occupancy) must not change the state,
+    and every tile shape at every occupancy must give the same bits."""
     wl = W.porous(256, seed=5, nc=320)
-    say("case autotune and tile shapes (row segments, wet runs), porous 256x320 FP64/FP32/FP16")
+    say("case autotune, tile shapes (row segments, wet runs) and occupancies, porous 256x320 FP64/FP32/FP16")
     for storage in (L.F64, L.F32_SHIFTED, L.F16_SHIFTED):
         ref = wl.make_lattice(storage)
         ref.step(80)
         for seg in (1, 2, 4, 8, L.LAUNCH_WET_RUNS, L.LAUNCH_RUN_TABLE):
-            lat = wl.make_lattice(storage)
-            lat.step(30)
-            before = lat.populations()
-            if seg == 1:
-                chosen, ms = lat.autotune(steps=2)
-                assert chosen in (1, 2, 4, 8, L.LAUNCH_WET_RUNS, L.LAUNCH_RUN_TABLE) and ms > 0
-            assert np.array_equal(before, lat.populations())
-            lat.set_launch(seg)
-            lat.step(50)
-            assert np.array_equal(lat.populations(), ref.populations()), (storage, seg)
-            lat.close()
+            for occ in (0, 7, 5, 3, 2):
+                lat = wl.make_lattice(storage)
+                lat.step(30)
+                before = lat.populations()
+                if seg == 1 and occ == 0:
+                    chosen, ms = lat.autotune(steps=2)
+                    assert chosen in (1, 2, 4, 8, L.LAUNCH_WET_RUNS, L.LAUNCH_RUN_TABLE) and ms > 0
+                    assert lat.occupancy() in (0, 7, 6, 5, 4, 3)
+                assert np.array_equal(before, lat.populations())
+                lat.set_launch(seg)
+                lat.set_occupancy(occ)
+                assert lat.occupancy() == occ
+                lat.step(50)
+                assert np.array_equal(lat.populations(), ref.populations()), (storage, seg, occ)
+                lat.close()
         ref.close()
+
+
+def test_occupancy_outside_the_range_is_rejected():
+    wl = W.porous(128, seed=5, nc=128)
+    lat = wl.make_lattice(L.F32_SHIFTED)
+    try:
+        for bad in (-1, 1, 8, 64):
+            with pytest.raises(L.LbmError) as e:
+                lat.set_occupancy(bad)
+            assert e.value.code == 1
+        assert lat.occupancy() == 0
+    finally:
+        lat.close()
 
 
 def test_new_geometry_under_the_tabled_shape():
