@@ -85,20 +85,26 @@ enum : int { BT_NONE = 0, BT_TOP, BT_RIGHT, BT_BOTTOM, BT_LEFT, BT_CORNER };
 // x + 16, ... of `chunk` consecutive blocks, so each XCD works through a few
 // neighbouring columns at a time (their row segments share its L2) and all
 // eight move through memory together.  Blocks past the last whole group of
-// 8 chunks keep their order.  The row-segment step kernels use chunks of 8
-// columns, the wet-run and tabled kernels 64 blocks, the others (ensembles,
-// set-up) 256 blocks.
+// 8 chunks keep their order.  Chunk sizes (step_chunk, kXcdChunkRuns,
+// kXcdChunk): the row-segment step kernels take one column per chunk when
+// a column holds at least 32 work-groups, else two; the wet-run and tabled
+// kernels 16 blocks (FP64) or 32 (FP32, FP16); the others (ensembles,
+// set-up) 1024 blocks.
 // Measured against one contiguous eighth of the tile order per XCD, one
 // process per variant, best launch shape per case: porous 8192^2 FP32 +5 %,
 // FP64 +5 %, FP16 +3 %; open 8192^2 FP32 +2 %, FP16 +5 %; 4096^2 cavity
-// FP32 +5 %, FP16 +6 %, FP64 +5 %; 512-lattice ensemble +3 %.  Chunks of
-// 256 blocks for the row segments too lost 3-4 % on the cavity's FP32 and
-// FP64 shapes with few blocks per column; for the wet runs 64 blocks beat
-// 256 by 1.3-1.9 % on the porous medium (all three formats).
+// FP32 +5 %, FP16 +6 %, FP64 +5 %; 512-lattice ensemble +3 %.  With the
+// blocked population layout and the tuned occupancy, these chunks then
+// beat the earlier 8 columns / 64 blocks / 256 blocks by 1-7 % (porous
+// 8192^2 FP32 +1 %, FP64 +6 %, FP16 +3 %; open 8192^2 FP32 +7 %; 4096^2
+// cavity FP64 +3 %, FP16 +1.6 %; ensemble +1 %).  Four or eight columns
+// per chunk were 2-4 % slower for the one-segment shapes; one column for
+// the cavity's 16 work-groups per column 1.5 % slower than two.
 // Block counts stay below 2^31 (the geometry caps the 64-row segments at
 // 2^30), so the arithmetic is 32-bit.
-constexpr uint32_t kXcdChunk = 256;
-constexpr uint32_t kXcdChunkRuns = 64;
+constexpr uint32_t kXcdChunk = 1024;
+template <typename T>
+constexpr uint32_t kXcdChunkRuns = sizeof(T) == 8 ? 16 : 32;
 
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb, uint32_t chunk = kXcdChunk)
 {
@@ -106,6 +112,14 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb, uint32_t 
     if (b >= whole) return b;
     const uint32_t x = b & 7, k = b >> 3;
     return ((k / chunk) * 8 + x) * chunk + (k % chunk);
+}
+
+// Chunk of the row-segment step kernels (wpc waves per column, four per
+// work-group): one column when it holds at least 32 work-groups, else two.
+__device__ __forceinline__ uint32_t step_chunk(int32_t wpc)
+{
+    const uint32_t per_column = (uint32_t)max(1, wpc / (kBlockRows / 64));
+    return per_column >= 32 ? per_column : 2 * per_column;
 }
 
 __device__ __forceinline__ int64_t edge_slot(int i, int j, int nr, int nc)
@@ -1342,7 +1356,7 @@ __device__ __forceinline__ void run_lane(const StepArgs &a, int i, int j, int s0
 template <typename T, bool GATHER, int R>
 __global__ __launch_bounds__(kBlockRows) void step_kernel(StepArgs a)
 {
-    const int64_t b = xcd_remap(blockIdx.x, (uint32_t)a.nblocks, 8u * (uint32_t)max(1, a.wpc / (kBlockRows / 64)));
+    const int64_t b = xcd_remap(blockIdx.x, (uint32_t)a.nblocks, step_chunk(a.wpc));
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int64_t gw = b * (kBlockRows / 64) + wave;
@@ -1360,7 +1374,7 @@ __global__ __launch_bounds__(kBlockRows) void run_kernel(StepArgs a)
     __shared__ int32_t rows[kBlockRows / 64][64];
     // every kernel argument the prologue needs, in one round trip
     asm volatile("" ::"s"(a.nblocks), "s"(a.nruns), "s"(a.runs), "s"(a.g.wet), "s"(a.g.sb), "s"(a.g.wl));
-    const int64_t b = xcd_remap(blockIdx.x, (uint32_t)a.nblocks, kXcdChunkRuns);
+    const int64_t b = xcd_remap(blockIdx.x, (uint32_t)a.nblocks, kXcdChunkRuns<T>);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t t = b * (kBlockRows / 64) + wave;
     if (t >= a.nruns) return;
@@ -1381,7 +1395,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlockRows) void tab_kernel(StepArgs a)
 {
     asm volatile("" ::"s"(a.nblocks), "s"(a.nruns), "s"(a.trun), "s"(a.tlane));  // one round trip
-    const int64_t b = xcd_remap(blockIdx.x, (uint32_t)a.nblocks, kXcdChunkRuns);
+    const int64_t b = xcd_remap(blockIdx.x, (uint32_t)a.nblocks, kXcdChunkRuns<T>);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t t = b * (kBlockRows / 64) + wave;
     if (t >= a.nruns) return;

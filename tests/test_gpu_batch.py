@@ -67,6 +67,29 @@ def test_ensemble_equals_lattices_alone(storage, launch, layouts):
     assert worst == 0.0
 
 
+@pytest.mark.parametrize("launch", ["tiles", "wet-runs"])
+def test_large_ensemble_equals_lattices_alone(launch):
+    """Enough tiles (and wet runs) that the work-groups are dealt to the XCDs
+    in chunks of 1024 (the remapping is the identity below 8 x 1024 groups):
+    every sample still ends where it ends alone, bit for bit."""
+    wls = small_ensemble(n=5, seed=23, sizes=(960, 1024))
+    tiles = sum((w.nr + 63) // 64 * w.nc for w in wls)
+    say(f"case large ensemble of {len(wls)} ({', '.join(w.name for w in wls)}), {tiles} tiles, FP32, {launch}")
+    assert tiles // 4 >= 2 * 8 * 1024
+    lats = [w.make_lattice(L.F32_SHIFTED) for w in wls]
+    with L.Batch(lats) as b:
+        if launch == "wet-runs":
+            b.set_launch(L.LAUNCH_WET_RUNS, 0)
+        else:
+            b.set_launch(1, 0)
+        b.step(1)
+        b.step(40, L.STEP_CHECK)
+    for w, lat in zip(wls, lats):
+        ref = run_alone(w, L.F32_SHIFTED, [41])
+        assert max_abs_wet(lat.populations(), ref, w.node_class) == 0.0, w.name
+        lat.close()
+
+
 @pytest.mark.parametrize("segments", [1, 2, 4, 8])
 def test_every_tile_shape_gives_the_same_result(segments):
     wls = small_ensemble(n=4, seed=17)
