@@ -141,9 +141,9 @@ def build_workload(name):
 
 def measured_traffic(key, kernel="step_kernel"):
     """HBM bytes per launch of a step kernel for a workload/storage/tile-shape
-    key, from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
-    profiles/ (calibrated against a plain copy kernel); None when that
-    kernel was not profiled."""
+    (/occupancy) key, from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    summarised in profiles/ (calibrated against a plain copy kernel); None
+    when that kernel was not profiled."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
@@ -194,7 +194,7 @@ def time_storage(wl, name, steps, warmup, ev):
                      ms_per_step=ms / steps, wall_s=wall, alg_bytes_per_step=alg, achieved_gbs=achieved,
                      frac=achieved / HBM_PEAK_GBS, segments=segments, occupancy=lat.occupancy(),
                      layout="blocked" if lat.layout() == L.LAYOUT_BLOCKED else "planes",
-                     traffic_bytes_per_step=measured_traffic(f"{wl.name}/{name}/R{segments}"))
+                     traffic_bytes_per_step=measured_traffic(f"{wl.name}/{name}/R{segments}/o{lat.occupancy()}"))
 
 
 def physical_cores():
