@@ -276,11 +276,11 @@ __device__ __forceinline__ Idx lane_idx(const Nbhd &h, int lane)
 // Element access through a wave-uniform base pointer and a 32-bit byte
 // offset, the addressing form global loads/stores take natively (scalar
 // base + vector offset).  The context only accepts lattices whose
-// population offsets (up to three planes) stay below 2^32 bytes.  The
-// accesses go through global-address-space pointers: a plane pointer read
-// from memory (the batch kernels' per-sample arguments) is otherwise a
-// generic pointer, and generic accesses become flat instructions with
-// 64-bit vector addresses.
+// population offsets stay below 2^32 bytes (the whole buffer blocked, up to
+// three planes in planes).  The accesses go through global-address-space
+// pointers: a direction's base pointer read from memory (the batch
+// kernels' per-sample arguments) is otherwise a generic pointer, and
+// generic accesses become flat instructions with 64-bit vector addresses.
 template <typename T>
 __device__ __forceinline__ const LBM_GLOBAL T *gaddr(const void *base, uint32_t byte_off)
 {
@@ -324,8 +324,9 @@ __device__ __forceinline__ uint32_t slot_bytes(const StepArgs &a, uint32_t n)
 // Streams the 9 populations of an interior wet node: one load per
 // direction, its address chosen between the neighbour (stream) and the
 // node's own opposite slot (bounce-back).  Opposite directions are two
-// planes apart, so both candidates are reached from the lower plane of the
-// pair with a non-negative 32-bit offset.
+// direction strides apart (two_planes bytes), so both candidates are
+// reached from the base of the pair's lower direction with a non-negative
+// 32-bit offset.
 template <typename T, typename WetFn>
 __device__ __forceinline__ void gather_interior_by(const StepArgs &a, const Idx &x, WetFn wet_src, T *p)
 {
@@ -336,7 +337,7 @@ __device__ __forceinline__ void gather_interior_by(const StepArgs &a, const Idx 
     for (int d = 1; d < 9; ++d) {
         const bool wet = wet_src(d);
         const int o = kOpp[d];
-        const bool low = d < o;  // d is the lower plane of its pair
+        const bool low = d < o;  // d is the lower direction of its pair
         const T *base = (const T *)a.fin_plane[low ? d : o];
         uint32_t nb = slot_bytes<T>(a, (uint32_t)x.src[d]) + (low ? 0u : twoP);
         uint32_t ob = own + (low ? twoP : 0u);
@@ -420,7 +421,7 @@ __device__ __forceinline__ void gather_edge(const StepArgs &a, const T *__restri
 
 // Writes the 9 post-collision populations of the node with compact index n.
 // Each store helper takes its own opaque copy of the byte offset (empty
-// asm): the compiler then cannot hoist the nine plane addresses shared by
+// asm): the compiler then cannot hoist the nine direction addresses shared by
 // the plain and the streaming branch into 64-bit vector pointers, and keeps
 // every store in the scalar-base + 32-bit-offset form.
 template <typename T>
@@ -1097,7 +1098,7 @@ __device__ __forceinline__ bool perimeter_segment(const Geometry &g, int j, int6
 // join, and the merge waits for the data, which serialised the segments
 // (one round trip to memory per segment instead of one per wave).  Dry
 // lanes therefore read their own slot (bounce-back address, always in
-// range: a plane holds at least one slot more than the lattice uses), and
+// range: the compact range holds one slot more than the lattice uses), and
 // segments the wave does not compute (perimeter, all dry, past the
 // column's end) read slot 0.  Collision and stores then run per segment
 // under the wet lanes' mask.  Keeping the general path out of this kernel
@@ -1154,7 +1155,7 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
                      "+v"(raw[0][5]), "+v"(raw[0][6]), "+v"(raw[0][7]), "+v"(raw[0][8]));
     if constexpr (sizeof(T) == 8) __builtin_amdgcn_s_setprio(0);
     if constexpr (sizeof(T) == 4 && R == 2) {
-        // FP32, two segments: collide both, then store plane by plane
+        // FP32, two segments: collide both, then store direction by direction
         // across them.  The cache line shared by the two segments' ranges
         // then gets both of its parts back to back instead of nine stores
         // apart, by which time the wave's share of L2 may have evicted the
@@ -1526,7 +1527,7 @@ __global__ __launch_bounds__(kBlockRows) void batch_kernel(BatchArgs bt)
         if (t >= bt.ntiles) return;
         const TileRef tr = tile_at(bt.tiles, t);
         const StepArgs a = args_at(bt, tr.s);
-        // the plane pointers are loaded with the sample's other arguments,
+        // the direction pointers are loaded with the sample's other arguments,
         // not after its mask words arrive
         asm volatile("" ::"s"(a.fin_plane[0]), "s"(a.fin_plane[1]), "s"(a.fin_plane[2]), "s"(a.fin_plane[5]),
                      "s"(a.fin_plane[6]), "s"(a.block_mask), "s"(a.two_planes));
