@@ -215,6 +215,26 @@ def test_graph_replay_equals_eager_steps():
     orc.close()
 
 
+@pytest.mark.parametrize("storage", [L.F32_SHIFTED, L.F16_SHIFTED])
+def test_occupancy_changes_between_graph_replays(storage):
+    """Captured step loops are rebuilt when the occupancy (or the shape)
+    changes: replays at one occupancy, then another, end bit for bit where
+    eager launches end, with downloads and reductions in between."""
+    wl = W.porous(300, seed=7, nc=260)
+    say(f"case occupancy and shape changes between graph replays, porous 300x260, storage {storage}")
+    g = wl.make_lattice(storage)
+    e = wl.make_lattice(storage)
+    for occ, shape in ((6, 1), (3, L.LAUNCH_WET_RUNS), (0, 2), (7, L.LAUNCH_RUN_TABLE)):
+        g.set_occupancy(occ)
+        g.set_launch(shape)
+        g.step(70)
+        e.step(70, L.STEP_NO_GRAPH)
+        assert np.array_equal(g.populations(), e.populations()), (occ, shape)
+        assert np.array_equal(g.reduce(), e.reduce()), (occ, shape)
+    for lat in (g, e):
+        lat.close()
+
+
 def test_autotune_keeps_state_and_results():
     """The autotuner (launch shape and occupancy) must not change the state,
     and every tile shape at every occupancy must give the same bits."""
