@@ -309,8 +309,10 @@ int lbm_get_launch(lbm_ctx *ctx, int32_t *segments);
 /* Occupancy of the step kernels: at most `groups_per_cu` work-groups (of
  * four waves) per compute unit, 2 to 7, or 0 for as many as their
  * registers allow (8 for most shapes).  Fewer waves in flight can move more
- * bytes per second (MI355X, FP64 and FP32 storage).  Results are the same
- * bits at every occupancy.  No counterpart in the reference. */
+ * bytes per second.  Until set or tuned it is 7 (FP32, FP16 storage) or 5
+ * (FP64), the best for the default shapes on MI355X; lbm_get_occupancy
+ * reports the value in use.  Results are the same bits at every occupancy.
+ * No counterpart in the reference. */
 int lbm_set_occupancy(lbm_ctx *ctx, int32_t groups_per_cu);
 int lbm_get_occupancy(lbm_ctx *ctx, int32_t *groups_per_cu);
 

@@ -62,7 +62,7 @@ StepArgs make_args(const lbm_ctx *c)
     a.segs = (int32_t)c->segs;
     a.esize = (int32_t)c->esize;
     a.block_mask = c->blocked ? ~63u : 0u;
-    a.launch_lds = occupancy_lds(c->occupancy);
+    a.launch_lds = occupancy_lds(c->occupancy >= 0 ? c->occupancy : default_occupancy(c->storage));
     // The reference recomputes t = 1.0 / tau inside every call
     // (BGKCollision.hpp:140-141) and (1.0/tau_even) etc. inside every
     // expression (TRTCollision.hpp:155-188); IEEE division makes these the
@@ -1080,7 +1080,7 @@ int lbm_set_occupancy(lbm_ctx *c, int32_t groups_per_cu)
 int lbm_get_occupancy(lbm_ctx *c, int32_t *groups_per_cu)
 {
     if (!c || !groups_per_cu) return fail(LBM_ERR_INVALID, "null argument");
-    *groups_per_cu = c->occupancy;
+    *groups_per_cu = c->occupancy >= 0 ? c->occupancy : default_occupancy(c->storage);
     return LBM_OK;
 }
 

@@ -41,7 +41,7 @@ struct lbm_ctx {
     bool has_geom = false;
     int64_t nr = 0, nc = 0, ldr = 0, plane = 0, wl = 0, E = 0, segs = 0;
     bool blocked = false;  // population layout: blocked (true) or 9 planes
-    int32_t occupancy = 0;  // step kernels: work-groups per CU (2..7), 0 = no limit
+    int32_t occupancy = -1;  // step kernels: work-groups per CU (2..7), 0 = no limit, -1 = default_occupancy
     uint64_t *d_wet = nullptr;
     std::vector<uint64_t> h_wet;  // host copy of the wet words (tile tables)
     std::vector<int32_t> h_sb;    // host copy of the segment bases

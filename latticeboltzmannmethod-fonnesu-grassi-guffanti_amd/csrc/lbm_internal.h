@@ -157,6 +157,8 @@ hipError_t launch_batch_reduce(int32_t storage, bool gather, int32_t R, const Ba
 
 // Segments of 64 rows per wave the step kernels use by default.
 int32_t default_segments(int32_t storage);
+// Occupancy of the step kernels until one is set or tuned (work-groups per CU).
+int32_t default_occupancy(int32_t storage);
 
 // Kernel launchers (lbm_kernels.hip).  All are asynchronous on `s`.
 // R: 64-row segments per wave (1, 2, 4 or 8; <= 0 = default_segments).

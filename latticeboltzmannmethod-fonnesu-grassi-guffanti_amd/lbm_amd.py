@@ -386,7 +386,8 @@ class Lattice:
         return r.value
 
     def set_occupancy(self, groups_per_cu):
-        """Work-groups per compute unit of the step kernels (2..7), 0 = no limit."""
+        """Work-groups per compute unit of the step kernels (2..7), 0 = no
+        limit; until set or tuned, the storage format's default (occupancy())."""
         _check(lib().lbm_set_occupancy(self._h, groups_per_cu))
 
     def occupancy(self):

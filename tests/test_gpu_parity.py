@@ -262,17 +262,25 @@ def test_autotune_keeps_state_and_results():
         ref.close()
 
 
-def test_occupancy_outside_the_range_is_rejected():
+def test_occupancy_defaults_and_range():
+    """Until set or tuned the occupancy is the storage format's default (7
+    work-groups per CU for FP32 and FP16, 5 for FP64); 0 (no limit) and 2..7
+    are accepted, anything else is rejected and leaves it unchanged."""
     wl = W.porous(128, seed=5, nc=128)
-    lat = wl.make_lattice(L.F32_SHIFTED)
-    try:
-        for bad in (-1, 1, 8, 64):
-            with pytest.raises(L.LbmError) as e:
-                lat.set_occupancy(bad)
-            assert e.value.code == 1
-        assert lat.occupancy() == 0
-    finally:
-        lat.close()
+    for storage, default in ((L.F64, 5), (L.F32_SHIFTED, 7), (L.F16_SHIFTED, 7)):
+        lat = wl.make_lattice(storage)
+        try:
+            assert lat.occupancy() == default
+            for bad in (-1, 1, 8, 64):
+                with pytest.raises(L.LbmError) as e:
+                    lat.set_occupancy(bad)
+                assert e.value.code == 1
+            assert lat.occupancy() == default
+            for ok in (0, 2, 7):
+                lat.set_occupancy(ok)
+                assert lat.occupancy() == ok
+        finally:
+            lat.close()
 
 
 def test_new_geometry_under_the_tabled_shape():
