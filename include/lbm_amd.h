@@ -323,7 +323,9 @@ int lbm_get_occupancy(lbm_ctx *ctx, int32_t *groups_per_cu);
  * pair.  Reports the shape and its time per step (outputs may be NULL);
  * lbm_get_occupancy reports the occupancy.  LBM_ERR_STATE for a context
  * that belongs to an ensemble (the ensemble has its own,
- * lbm_batch_autotune).  No counterpart in the reference. */
+ * lbm_batch_autotune) and for a lattice with partially saturated
+ * bounce-back (every launch advances its carried obstacle values).  No
+ * counterpart in the reference. */
 int lbm_autotune(lbm_ctx *ctx, int32_t steps, int32_t *segments, double *ms_per_step);
 
 /* ------------------------------------------------------------------ */

@@ -1097,6 +1097,12 @@ int lbm_autotune(lbm_ctx *c, int32_t steps, int32_t *segments, double *ms_per_st
     if (steps < 1) return fail(LBM_ERR_INVALID, "steps must be >= 1");
     if (c->batch) return fail(LBM_ERR_STATE, "context belongs to an ensemble: tune it with lbm_batch_autotune");
     if (c->state == kNone) return fail(LBM_ERR_STATE, "no populations (call lbm_init_equilibrium)");
+    // Every launch of a partially saturated bounce-back lattice rewrites the
+    // carried obstacle values (ps_val) in place, so a timing launch would
+    // change the state.
+    if (c->psbb)
+        return fail(LBM_ERR_STATE, "autotune: a lattice with partially saturated bounce-back runs one general "
+                                   "kernel and would change its carried obstacle values; set the occupancy by hand");
     HIP_TRY(hipSetDevice(c->desc.device));
     {
         // the tabled-run candidate's tables, before the arguments that point

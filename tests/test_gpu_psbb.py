@@ -135,6 +135,13 @@ def test_rules():
     with pytest.raises(L.LbmError) as e:
         L.Batch([lat2])
     assert e.value.code == L.ERR_UNSUPPORTED
+    lat2.step(3)
+    before = lat2.populations()
+    with pytest.raises(L.LbmError) as e:
+        lat2.autotune(steps=2)                        # timing launches would advance the carried values
+    assert e.value.code == L.ERR_STATE
+    lat2.set_occupancy(4)
+    assert np.array_equal(lat2.populations(), before)
     bad = np.ones(int(lat2.node_counts()[6]))
     bad[0] = np.nan
     with pytest.raises(L.LbmError):
