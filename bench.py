@@ -311,6 +311,24 @@ def bench_cavity_trt(args, ev):
                 u_lid=wl.info["u_lid"], storages=results, error_vs_f64=errors)
 
 
+def bench_open(args, ev):
+    """The porous workload's 8192 x 8192 channel with no obstacles (67.1M wet
+    nodes, same pressure-driven BGK set-up): MLUPS and roofline fraction of
+    the FP64, FP32 and FP16 storage formats over the headline's step count;
+    timing only (its errors are not stated)."""
+    wl = build_workload("open8192")
+    log(f"open8192: {wl.wet_count()} wet nodes")
+    results = {}
+    for s in ("f64", "f32", "f16"):
+        lat, r = time_storage(wl, s, args.steps, args.warmup, ev)
+        results[s] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+        lat.close()
+        log(f"  open8192 {s}: {r['mlups']:.1f} MLUPS, {r['ms_per_step']:.4f} ms/step, "
+            f"{r['achieved_gbs']:.0f} GB/s = {100 * r['frac']:.1f}% of 8 TB/s "
+            f"(launch {r['segments']}, occupancy {r['occupancy'] or 'full'}, {r['layout']})")
+    return dict(workload=wl.name, wet_nodes=wl.wet_count(), collision="BGK", storages=results)
+
+
 def bench_ensemble(args, ev):
     """Many independent porous lattices of uneven sizes advanced by one
     persistent launch per step (lbm_batch), HIP-graph step loop, tile shape
@@ -384,6 +402,7 @@ def main():
                     help="steps after which the ensemble's FP32 permeabilities are checked against FP64 (0 = skip)")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-c4", action="store_true", help="skip the 4096^2 TRT cavity line")
+    ap.add_argument("--no-open", action="store_true", help="skip the open 8192^2 channel line")
     ap.add_argument("--c4-error-steps", type=int, default=200000,
                     help="steps after which the 4096^2 TRT cavity's FP32/FP16 errors are stated (0 = skip)")
     ap.add_argument("--no-patterns", action="store_true", help="skip the access-pattern ceilings")
@@ -445,6 +464,7 @@ def main():
         lat.close()
     ens = None if args.no_ensemble else bench_ensemble(args, ev)
     c4 = None if args.no_c4 else bench_cavity_trt(args, ev)
+    opn = None if args.no_open or args.workload == "open8192" else bench_open(args, ev)
     cpu = None
     if not args.no_cpu_baseline:
         log("  CPU baseline (reference-structured oracle) ...")
@@ -477,6 +497,7 @@ def main():
         "error_vs_f64": errors,
         "ensemble": ens,
         "cavity_trt4096": c4,
+        "open8192": opn,
     }
     if "f64" in results and head != "f64":
         line["speedup_vs_f64"] = round(r["mlups"] / results["f64"]["mlups"], 3)
