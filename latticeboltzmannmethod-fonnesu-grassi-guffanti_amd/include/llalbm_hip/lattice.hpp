@@ -96,6 +96,7 @@ public:
         if (ctx) {
             lbm_destroy(ctx);
             ctx = nullptr;
+            tuned = false;
         }
         build_lists();
         host_valid = false;
@@ -142,6 +143,14 @@ public:
         };
         if (n_steps > 0) push_values(0, true, [] {});
         lbm_check(lbm_init_equilibrium(ctx), "lbm_init_equilibrium");
+        // the launch shape and occupancy, timed on the initial state (the
+        // timing launches write only the spare buffer), before any
+        // partially saturated bounce-back is attached (lbm_autotune refuses
+        // those lattices)
+        if (tune_steps > 0 && n_steps > 0 && !tuned) {
+            lbm_check(lbm_autotune(ctx, tune_steps, nullptr, nullptr), "lbm_autotune");
+            tuned = true;
+        }
         ParallelizationPolicy::setup_obstacles(ctx, obstacle_nodes);
         saves.reset(lattice_dimensions[0], lattice_dimensions[1]);
         auto t0 = std::chrono::high_resolution_clock::now();
@@ -263,6 +272,12 @@ public:
     /// Lattice.hpp:634-637.
     void attach_flow_analyzer(std::shared_ptr<analysis::FlowAnalyzer<2>> fa) { flow_analyzer = fa; }
 
+    /// Launch-shape and occupancy tuning (no counterpart in the reference):
+    /// with steps > 0, the next perform_lbm times every candidate over that
+    /// many launches on its initial state and keeps the fastest (once per
+    /// context; results are the same bits).  0 turns it off.
+    void set_autotune(int steps = 5) { tune_steps = steps; }
+
     /// Storage of the populations (LBM_STORE_F64 is the reference-parity path).
     void set_storage(int32_t storage_)
     {
@@ -270,6 +285,7 @@ public:
         if (ctx) {
             lbm_destroy(ctx);
             ctx = nullptr;
+            tuned = false;
         }
         host_valid = false;
         ran = false;
@@ -308,6 +324,7 @@ private:
             std::string msg = lbm_last_error();
             lbm_destroy(ctx);
             ctx = nullptr;
+            tuned = false;
             throw LbmError(r, "lbm_set_geometry: " + msg);
         }
     }
@@ -470,6 +487,8 @@ private:
     bool ran = false;  // a run has completed since the geometry was set
     SaveQueue saves;
     int32_t storage = LBM_STORE_F64;
+    int tune_steps = 0;  // set_autotune
+    bool tuned = false;  // the context's shape and occupancy have been tuned
     lbm_ctx *ctx = nullptr;
     std::shared_ptr<analysis::FlowAnalyzer<2>> flow_analyzer;
     double last_elapsed = 0.0;

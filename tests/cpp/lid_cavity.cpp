@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <fstream>
 #include <iostream>
+#include <string>
 
 int main(int argc, char **argv)
 {
@@ -14,6 +15,7 @@ int main(int argc, char **argv)
     using namespace llalbm::util::generation;
     const char *out_path = argc > 1 ? argv[1] : "populations.bin";
     const int steps = argc > 2 ? std::atoi(argv[2]) : 500;
+    const bool tune = argc > 3 && std::string(argv[3]) == "tune";  // launch shape and occupancy tuned first
 
     using Configuration = LatticeConfiguration<2, collisions::HIPTRTCollisionPolicy<2>, boundaries::HIPBounceBackPolicy<2>,
                                                boundaries::HIPBounceBackPolicy<2>, boundaries::HIPZouHePolicy<2>,
@@ -46,6 +48,7 @@ int main(int argc, char **argv)
     analysis->set_iterations_between_save(100);
     lattice.attach_flow_analyzer(analysis);
 
+    if (tune) lattice.set_autotune(3);
     lattice.perform_lbm(steps, 1, 100, false);
 
     auto &pop = lattice.get_populations();

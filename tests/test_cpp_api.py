@@ -42,12 +42,16 @@ def _cavity_with_disc():
 
 
 @pytest.mark.gpu
-def test_cpp_driver_matches_oracle(tmp_path):
+@pytest.mark.parametrize("tune", ["", "tune"])
+def test_cpp_driver_matches_oracle(tmp_path, tune):
+    """tune: Lattice::set_autotune before perform_lbm (the shape and
+    occupancy change no bit)."""
     exe = os.path.join(CPP_DIR, "lid_cavity")
     assert os.path.exists(exe), "tests/cpp/lid_cavity is built by __graft_entry__.build()"
     steps = 500
     out = tmp_path / "pop.bin"
-    r = subprocess.run([exe, str(out), str(steps)], capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    r = subprocess.run([exe, str(out), str(steps)] + ([tune] if tune else []), capture_output=True, text=True,
+                       cwd=tmp_path, timeout=120)
     assert r.returncode == 0, r.stderr[-2000:]
     lines = [ln.split() for ln in r.stdout.splitlines() if ln.startswith("lift_drag")]
     cpp_ld = np.array([[float(a), float(b)] for _, a, b in lines])
