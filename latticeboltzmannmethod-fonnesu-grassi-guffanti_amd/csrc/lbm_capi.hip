@@ -772,8 +772,13 @@ int lbm_set_geometry(lbm_ctx *c, const uint8_t *cls)
     // Wet runs of the run kernel: in every interior column, the wet rows of
     // the segments the perimeter kernel does not take (the first segment
     // when row 0 is wet, the one holding row NR-1 when that row is wet),
-    // cut into runs of at most 64 rows that span at most 128 rows.  Entry:
-    // column, first row, its compact index, number of rows.
+    // cut into runs of at most 64 rows that span at most 128 rows and never
+    // cross a multiple of 64 in the compact index (a block of the blocked
+    // layout).  A wave's own-column loads and all its stores then fall in
+    // one block per direction instead of two partial ones: on the 8192^2
+    // porous medium this measured FP32 +7 %, FP64 +5 %, FP16 (tabled) +3 %
+    // at each format's best launch, the same bits.  Entry: column, first
+    // row, its compact index, number of rows.
     {
         std::vector<int32_t> runs;
         auto push = [&](int64_t j, int64_t i, int64_t s0, int64_t n) {
@@ -793,7 +798,7 @@ int lbm_set_geometry(lbm_ctx *c, const uint8_t *cls)
                 while (m) {
                     const int64_t i = 64 * k + __builtin_ctzll(m);
                     m &= m - 1;
-                    if (n == 64 || (n > 0 && i >= i_first + 128)) {
+                    if (n == 64 || (n > 0 && (i >= i_first + 128 || (slot & 63) == 0))) {
                         push(j, i_first, s_first, n);
                         n = 0;
                     }

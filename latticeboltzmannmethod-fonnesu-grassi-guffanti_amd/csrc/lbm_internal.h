@@ -81,8 +81,9 @@ struct StepArgs {
     int32_t *flag;          // bit 0: NaN density seen
     // Wet-run table of the run kernel (kRunLaunch): per run, 4 int32 --
     // column, first row, compact index of the first row, wet rows (<= 64,
-    // all within 128 rows of the first); only rows of segments the
-    // perimeter kernel does not take.
+    // all within 128 rows of the first and within one 64-node block of the
+    // compact index); only rows of segments the perimeter kernel does not
+    // take.
     const int32_t *runs;
     int64_t nruns;
     // Perimeter items of the perimeter kernel: per 64-row segment holding a

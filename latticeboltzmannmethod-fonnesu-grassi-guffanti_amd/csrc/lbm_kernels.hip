@@ -1211,7 +1211,8 @@ __device__ __forceinline__ void step_wave(const StepArgs &a, int j, int64_t row0
 
 // One wave's share of a gathering step in the run layout: run t of the
 // wet-run table, up to 64 consecutive wet rows of column j from row i0,
-// all within 128 rows.  Lane l takes the l-th of them, so in porous media
+// all within 128 rows and within one 64-node block of the compact index.
+// Lane l takes the l-th of them, so in porous media
 // no lane idles on a solid node (a 64-row segment leaves those lanes
 // empty).  A lane's own compact index is the run's first index plus l.
 // Its row comes from the column's mask words: each wet row of the window
