@@ -88,8 +88,9 @@ enum : int { BT_NONE = 0, BT_TOP, BT_RIGHT, BT_BOTTOM, BT_LEFT, BT_CORNER };
 // 8 chunks keep their order.  Chunk sizes (step_chunk, kXcdChunkRuns,
 // kXcdChunk): the row-segment step kernels take one column per chunk when
 // a column holds at least 32 work-groups, else two; the wet-run and tabled
-// kernels 16 blocks (FP64) or 32 (FP32, FP16); the others (ensembles,
-// set-up) 1024 blocks.
+// kernels 16 blocks (FP64) or 32 (FP32, FP16; the two-run FP16 tabled
+// waves 16, the same 128 runs); the others (ensembles, set-up) 1024
+// blocks.
 // Measured against one contiguous eighth of the tile order per XCD, one
 // process per variant, best launch shape per case: porous 8192^2 FP32 +5 %,
 // FP64 +5 %, FP16 +3 %; open 8192^2 FP32 +2 %, FP16 +5 %; 4096^2 cavity
@@ -1397,33 +1398,76 @@ __global__ __launch_bounds__(kBlockRows) void run_kernel(StepArgs a)
 // The wet runs with every lane's addressing read from the tables built with
 // the geometry (StepArgs::trun, tlane): the run entry (scalar) and the
 // lane's record (one 4-byte vector load) are the only loads before the
-// population loads, and they issue together.  (Two runs per wave, the loads
-// of both in flight together, measured no faster for FP16 and slower for
-// FP32 on the 8192^2 porous medium: the wave count per second is not
-// limited by the bytes in flight.)
+// population loads, and they issue together.  An FP16 wave takes two
+// consecutive runs and issues the loads of both before either collides
+// (kTabRunsPerWave); its XCD chunks are halved so that a chunk still holds
+// the same runs.  With the runs cut at the layout's blocks this measured
+// +7 % for FP16 on the 8192^2 porous medium (same bits); before, when runs
+// straddled the blocks, it was no faster, and for FP32 it was slower.
+template <typename T>
+constexpr int kTabRunsPerWave = sizeof(T) == 2 ? 2 : 1;
+
+// One tabled run's lane: row, neighbour ranks and source-is-wet bits from
+// the lane's record (see StepArgs::tlane), then the gather.
+struct TabLane {
+    int i, j, s0, n;
+    bool act;
+    int32_t rank0, rank2;
+    uint32_t wet;
+};
+
+__device__ __forceinline__ TabLane tab_lane(const StepArgs &a, int64_t t, int lane)
+{
+    const LBM_CONST int32_t *tp = (const LBM_CONST int32_t *)a.trun + 8 * t;
+    const uint32_t rec = ((const LBM_GLOBAL uint32_t *)a.tlane)[64 * t + lane];
+    TabLane r;
+    r.j = __builtin_amdgcn_readfirstlane(tp[0]);
+    const int i0 = __builtin_amdgcn_readfirstlane(tp[1]);
+    r.s0 = __builtin_amdgcn_readfirstlane(tp[2]);
+    r.n = __builtin_amdgcn_readfirstlane(tp[3]);
+    const int32_t bl = __builtin_amdgcn_readfirstlane(tp[4]);
+    const int32_t br = __builtin_amdgcn_readfirstlane(tp[5]);
+    r.act = lane < r.n;
+    r.i = i0 + (int)(rec & 127u);
+    r.rank0 = bl + (int32_t)((rec >> 7) & 127u);
+    r.rank2 = br + (int32_t)((rec >> 14) & 127u);
+    r.wet = r.act ? ((rec >> 21) & 255u) << 1 : 0u;
+    return r;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlockRows) void tab_kernel(StepArgs a)
 {
     asm volatile("" ::"s"(a.nblocks), "s"(a.nruns), "s"(a.trun), "s"(a.tlane));  // one round trip
-    const int64_t b = xcd_remap(blockIdx.x, (uint32_t)a.nblocks, kXcdChunkRuns<T>);
+    const int64_t b = xcd_remap(blockIdx.x, (uint32_t)a.nblocks, kXcdChunkRuns<T> / kTabRunsPerWave<T>);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if constexpr (kTabRunsPerWave<T> == 2) {
+        // runs t0 and t0 + 1; a last odd run reads its own run twice and
+        // stores once
+        const int64_t t0 = 2 * (b * (kBlockRows / 64) + wave);
+        if (t0 >= a.nruns) return;
+        const bool two = t0 + 1 < a.nruns;
+        const int lane = threadIdx.x & 63;
+        const TabLane x = tab_lane(a, t0, lane);
+        TabLane y = tab_lane(a, two ? t0 + 1 : t0, lane);
+        y.act = y.act && two;
+        y.wet = two ? y.wet : 0u;
+        T rx[9], ry[9];
+        run_gather<T>(a, x.s0, lane, x.act, x.rank0, x.rank2, x.wet, rx);
+        run_gather<T>(a, y.s0, lane, y.act, y.rank0, y.rank2, y.wet, ry);
+        // all eighteen loads issue before any value is used (see run_lane)
+        asm volatile("" : "+v"(rx[0]), "+v"(rx[1]), "+v"(rx[2]), "+v"(rx[3]), "+v"(rx[4]), "+v"(rx[5]),
+                     "+v"(rx[6]), "+v"(rx[7]), "+v"(rx[8]), "+v"(ry[0]), "+v"(ry[1]), "+v"(ry[2]), "+v"(ry[3]),
+                     "+v"(ry[4]), "+v"(ry[5]), "+v"(ry[6]), "+v"(ry[7]), "+v"(ry[8]));
+        run_finish<T>(a, x.i, x.j, x.s0, x.n, lane, x.act, rx);
+        run_finish<T>(a, y.i, y.j, y.s0, y.n, lane, y.act, ry);
+        return;
+    }
     const int64_t t = b * (kBlockRows / 64) + wave;
     if (t >= a.nruns) return;
     const int lane = threadIdx.x & 63;
-    const LBM_CONST int32_t *tp = (const LBM_CONST int32_t *)a.trun + 8 * t;
-    const uint32_t rec = ((const LBM_GLOBAL uint32_t *)a.tlane)[64 * t + lane];
-    const int j = __builtin_amdgcn_readfirstlane(tp[0]);
-    const int i0 = __builtin_amdgcn_readfirstlane(tp[1]);
-    const int s0 = __builtin_amdgcn_readfirstlane(tp[2]);
-    const int n = __builtin_amdgcn_readfirstlane(tp[3]);
-    const int32_t bl = __builtin_amdgcn_readfirstlane(tp[4]);
-    const int32_t br = __builtin_amdgcn_readfirstlane(tp[5]);
-    const bool act = lane < n;
-    const int i = i0 + (int)(rec & 127u);
-    const int32_t rank0 = bl + (int32_t)((rec >> 7) & 127u);
-    const int32_t rank2 = br + (int32_t)((rec >> 14) & 127u);
-    const uint32_t wet = act ? ((rec >> 21) & 255u) << 1 : 0u;
-    run_lane<T>(a, i, j, s0, n, lane, act, rank0, rank2, wet);
+    const TabLane x = tab_lane(a, t, lane);
+    run_lane<T>(a, x.i, x.j, x.s0, x.n, lane, x.act, x.rank0, x.rank2, x.wet);
 }
 
 // The perimeter share of a gathering step of one lattice: one wave per
@@ -1887,7 +1931,8 @@ hipError_t launch_step_tab(StepArgs a, hipStream_t s)
 {
     hipError_t e = hipSuccess;
     if (a.nruns > 0 && (!a.trun || !a.tlane)) return hipErrorInvalidValue;  // tables not built
-    if (a.nruns > 0) e = launch(tab_kernel<T>, (a.nruns + kBlockRows / 64 - 1) / (kBlockRows / 64), s, a);
+    const int64_t per_block = (kBlockRows / 64) * kTabRunsPerWave<T>;
+    if (a.nruns > 0) e = launch(tab_kernel<T>, (a.nruns + per_block - 1) / per_block, s, a);
     if (e != hipSuccess) return e;
     return launch_perimeter<T>(a, s);
 }
